@@ -1,0 +1,398 @@
+"""Python host mirror of clBool's API over the MI355X C-ABI (libclbool_amd.so).
+
+Names and argument meaning follow the reference's include/clbool.hpp:
+``create_controls``, ``matrix_dcsr`` / ``matrix_coo`` / ``matrix_csr``, the
+conversions of src/common/matrices_conversions.hpp, ``dcsr.matrix_multiplication_hash``,
+``dcsr.kronecker_product``, ``coo.matrix_addition``, ``csr.matrix_addition``,
+``prefix_sum`` and ``fill_with``.  Operations return their output instead of
+filling an out-parameter.  Errors raise ``ClboolError`` carrying clBool's
+Status code (src/core/error.hpp:130-151).
+
+Every call goes to hand-written HIP kernels; nothing here computes on the CPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import types
+
+import numpy as np
+
+from . import _capi
+from ._capi import Coo as _Coo, Csr as _Csr, Dcsr as _Dcsr, SpgemmOpts, SpgemmStats, Tuning
+
+STATUS_NAMES = [
+    "CLBOOL_NO_SUCH_PROGRAM", "CLBOOL_CREATE_KERNEL_ERROR", "CLBOOL_BUILD_PROGRAM_ERROR",
+    "CLBOOL_INCOMPLETE_KERNEL", "CLBOOL_RUN_KERNEL_FAILURE", "CLBOOL_EVENT_WAITING_ERROR",
+    "CLBOOL_CREATE_BUFFER_ERROR", "CLBOOL_COPY_BUFFER_ERROR", "CLBOOL_READ_BUFFER_ERROR",
+    "CLBOOL_WRITE_BUFFER_ERROR", "CLBOOL_CREATE_CONTROLS_ERROR", "CLBOOL_INVALID_ARGUMENT",
+    "CLBOOL_INVALID_VALUE", "CLBOOL_INITIALIZATION_ERROR",
+]
+SPGEMM_PLAIN = _capi.SPGEMM_PLAIN
+SPGEMM_LEARNED = _capi.SPGEMM_LEARNED
+
+
+class ClboolError(RuntimeError):
+    """clbool::Exception: message plus Status."""
+
+    def __init__(self, status: int, message: str):
+        self.status = status
+        self.status_name = STATUS_NAMES[status] if 0 <= status < len(STATUS_NAMES) else str(status)
+        super().__init__(f"[{self.status_name}] {message}")
+
+
+def lib():
+    return _capi.load()
+
+
+def _check(rc: int):
+    if rc != 0:
+        msg = lib().clb_last_error()
+        raise ClboolError(rc - 1, msg.decode() if msg else "")
+
+
+def _u32(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.uint32))
+
+
+def _p(a):
+    return a.ctypes.data_as(_capi.u32p)
+
+
+# ---------------------------------------------------------------------------
+class Controls:
+    """clbool::Controls (src/core/controls.hpp:12-28): one device, one stream."""
+
+    def __init__(self, platform_id: int = 0, device_id: int = 0):
+        h = C.c_void_p()
+        _check(lib().clb_controls_create(platform_id, device_id, C.byref(h)))
+        self.handle = h
+        self.device_id = device_id
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().clb_controls_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sync(self):
+        _check(lib().clb_controls_sync(self.handle))
+
+    @property
+    def max_wg_size(self):
+        v = C.c_uint32()
+        _check(lib().clb_controls_max_wg_size(self.handle, C.byref(v)))
+        return v.value
+
+    def tuning(self) -> Tuning:
+        t = Tuning()
+        _check(lib().clb_get_tuning(self.handle, C.byref(t)))
+        return t
+
+    def set_tuning(self, t: Tuning):
+        _check(lib().clb_set_tuning(self.handle, C.byref(t)))
+
+    def last_stats(self) -> dict:
+        s = SpgemmStats()
+        _check(lib().clb_last_stats(self.handle, C.byref(s)))
+        return s.as_dict()
+
+    def trim(self):
+        _check(lib().clb_pool_trim(self.handle))
+
+
+def create_controls(platform_id: int = 0, device_id: int = 0) -> Controls:
+    """create_controls (src/common/env.cpp:7-36)."""
+    return Controls(platform_id, device_id)
+
+
+def device_count() -> int:
+    n = C.c_uint32()
+    _check(lib().clb_device_count(C.byref(n)))
+    return n.value
+
+
+def show_devices() -> list:
+    """show_devices (src/common/env.cpp:38-56), returned as dicts."""
+    out = []
+    for i in range(device_count()):
+        info = _capi.DeviceInfo()
+        _check(lib().clb_device_info_get(i, C.byref(info)))
+        out.append({"id": i, "name": info.name.decode(), "arch": info.arch.decode(),
+                    "compute_units": info.compute_units, "max_wg_size": info.max_wg_size,
+                    "lds_bytes": info.lds_bytes, "global_mem_bytes": info.global_mem_bytes,
+                    "mem_clock_khz": info.mem_clock_khz, "mem_bus_width": info.mem_bus_width})
+    return out
+
+
+class Buffer:
+    """A reference-counted device buffer (replaces cl::Buffer)."""
+
+    def __init__(self, handle, owned=True):
+        self.handle = C.c_void_p(handle) if not isinstance(handle, C.c_void_p) else handle
+        if not owned and self.handle:
+            lib().clb_buffer_retain(self.handle)
+
+    @classmethod
+    def create(cls, controls: Controls, nbytes: int) -> "Buffer":
+        h = C.c_void_p()
+        _check(lib().clb_buffer_create(controls.handle, nbytes, C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def from_array(cls, controls: Controls, arr) -> "Buffer":
+        arr = np.ascontiguousarray(arr)
+        h = C.c_void_p()
+        _check(lib().clb_buffer_from_host(controls.handle, arr.ctypes.data, arr.nbytes, C.byref(h)))
+        return cls(h)
+
+    @property
+    def nbytes(self):
+        return int(lib().clb_buffer_size(self.handle)) if self.handle else 0
+
+    @property
+    def device_ptr(self):
+        return lib().clb_buffer_device_ptr(self.handle)
+
+    def read(self, controls: Controls, count: int, dtype=np.uint32, offset=0):
+        out = np.empty(count, dtype=dtype)
+        if count:
+            _check(lib().clb_buffer_read(controls.handle, self.handle, offset, out.nbytes, out.ctypes.data))
+        return out
+
+    def write(self, controls: Controls, arr, offset=0):
+        arr = np.ascontiguousarray(arr)
+        _check(lib().clb_buffer_write(controls.handle, self.handle, offset, arr.nbytes, arr.ctypes.data))
+
+    def __del__(self):
+        try:
+            if self.handle:
+                lib().clb_buffer_release(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
+def _take(h):
+    return Buffer(h) if h else None
+
+
+def _raw(b):
+    return b.handle if b is not None else None
+
+
+# ---------------------------------------------------------------------------
+class matrix_dcsr:
+    """clbool::matrix_dcsr (src/core/matrix_dcsr.hpp:11-84)."""
+
+    def __init__(self, nrows=0, ncols=0, rpt=None, rows=None, cols=None, nnz=0, nzr=0):
+        self.nrows, self.ncols, self.nnz, self.nzr = nrows, ncols, nnz, nzr
+        self.rpt_gpu, self.rows_gpu, self.cols_gpu = rpt, rows, cols
+
+    def empty(self):
+        return self.nnz == 0
+
+    def _c(self):
+        return _Dcsr(_raw(self.rpt_gpu), _raw(self.rows_gpu), _raw(self.cols_gpu),
+                     self.nrows, self.ncols, self.nnz, self.nzr)
+
+    @classmethod
+    def _from_c(cls, m: _Dcsr):
+        return cls(m.nrows, m.ncols, _take(m.rpt), _take(m.rows), _take(m.cols), m.nnz, m.nzr)
+
+    def to_host(self, controls):
+        """matrix_dcsr_from_gpu (matrices_conversions.cpp:121-139): (rpt, rows, cols)."""
+        if self.nnz == 0:
+            return np.zeros(1, np.uint32), np.zeros(0, np.uint32), np.zeros(0, np.uint32)
+        return (self.rpt_gpu.read(controls, self.nzr + 1), self.rows_gpu.read(controls, self.nzr),
+                self.cols_gpu.read(controls, self.nnz))
+
+
+class matrix_coo:
+    """clbool::matrix_coo (src/core/matrix_coo.hpp:11-77)."""
+
+    def __init__(self, nrows=0, ncols=0, rows=None, cols=None, nnz=0):
+        self.nrows, self.ncols, self.nnz = nrows, ncols, nnz
+        self.rows_gpu, self.cols_gpu = rows, cols
+
+    def empty(self):
+        return self.nnz == 0
+
+    @classmethod
+    def from_host(cls, controls, rows, cols, nrows, ncols, nnz=None, sorted=True, noDuplicates=True):
+        """matrix_coo(controls, rows*, cols*, nrows, ncols, nnz, sorted, noDuplicates)."""
+        rows, cols = _u32(rows), _u32(cols)
+        nnz = rows.size if nnz is None else nnz
+        m = _Coo()
+        _check(lib().clb_coo_from_host(controls.handle, _p(rows), _p(cols), nrows, ncols, nnz,
+                                       int(sorted), int(noDuplicates), C.byref(m)))
+        return cls._from_c(m)
+
+    @classmethod
+    def from_buffers(cls, controls, nrows, ncols, nnz, rows: Buffer, cols: Buffer, sorted=True, noDuplicates=True):
+        m = _Coo()
+        _check(lib().clb_coo_from_buffers(controls.handle, nrows, ncols, nnz, _raw(rows), _raw(cols),
+                                          int(sorted), int(noDuplicates), C.byref(m)))
+        return cls._from_c(m)
+
+    def _c(self):
+        return _Coo(_raw(self.rows_gpu), _raw(self.cols_gpu), self.nrows, self.ncols, self.nnz)
+
+    @classmethod
+    def _from_c(cls, m: _Coo):
+        return cls(m.nrows, m.ncols, _take(m.rows), _take(m.cols), m.nnz)
+
+    def to_host(self, controls):
+        """matrix_coo_from_gpu (matrices_conversions.cpp:142-156): (rows, cols)."""
+        if self.nnz == 0:
+            return np.zeros(0, np.uint32), np.zeros(0, np.uint32)
+        return self.rows_gpu.read(controls, self.nnz), self.cols_gpu.read(controls, self.nnz)
+
+
+class matrix_csr:
+    """clbool::matrix_csr (src/core/matrix_csr.hpp:7-44)."""
+
+    def __init__(self, nrows=0, ncols=0, rpt=None, cols=None, nnz=0):
+        self.nrows, self.ncols, self.nnz = nrows, ncols, nnz
+        self.rpt_gpu, self.cols_gpu = rpt, cols
+
+    def empty(self):
+        return self.nnz == 0
+
+    def _c(self):
+        return _Csr(_raw(self.rpt_gpu), _raw(self.cols_gpu), self.nrows, self.ncols, self.nnz)
+
+    @classmethod
+    def _from_c(cls, m: _Csr):
+        return cls(m.nrows, m.ncols, _take(m.rpt), _take(m.cols), m.nnz)
+
+    def to_host(self, controls):
+        """matrix_csr_from_gpu (matrices_conversions.cpp:273-285): (rpt, cols)."""
+        if self.nnz == 0:
+            return np.zeros(self.nrows + 1, np.uint32), np.zeros(0, np.uint32)
+        return self.rpt_gpu.read(controls, self.nrows + 1), self.cols_gpu.read(controls, self.nnz)
+
+
+# ---------------------------------------------------------------------------
+def matrix_dcsr_from_cpu(controls, rpt, rows, cols, nrows, ncols=None) -> matrix_dcsr:
+    """matrix_dcsr_from_cpu (matrices_conversions.cpp:93-108); ncols defaults to nrows."""
+    rpt, rows, cols = _u32(rpt), _u32(rows), _u32(cols)
+    ncols = nrows if ncols is None else ncols
+    m = _Dcsr()
+    _check(lib().clb_dcsr_from_host(controls.handle, _p(rpt), _p(rows), _p(cols), rows.size, cols.size,
+                                    nrows, ncols, C.byref(m)))
+    return matrix_dcsr._from_c(m)
+
+
+def csr_from_cpu(controls, rpt, cols, nrows, ncols) -> matrix_csr:
+    """csr_from_cpu (matrices_conversions.cpp:199-208)."""
+    rpt, cols = _u32(rpt), _u32(cols)
+    m = _Csr()
+    _check(lib().clb_csr_from_host(controls.handle, _p(rpt), _p(cols), nrows, ncols, cols.size, C.byref(m)))
+    return matrix_csr._from_c(m)
+
+
+def coo_to_dcsr_shallow(controls, a: matrix_coo) -> matrix_dcsr:
+    m, ac = _Dcsr(), a._c()
+    _check(lib().clb_coo_to_dcsr_shallow(controls.handle, C.byref(ac), C.byref(m)))
+    return matrix_dcsr._from_c(m)
+
+
+def dcsr_to_coo_shallow(controls, a: matrix_dcsr) -> matrix_coo:
+    m, ac = _Coo(), a._c()
+    _check(lib().clb_dcsr_to_coo_shallow(controls.handle, C.byref(ac), C.byref(m)))
+    return matrix_coo._from_c(m)
+
+
+def dcsr_to_coo_deep(controls, a: matrix_dcsr) -> matrix_coo:
+    m, ac = _Coo(), a._c()
+    _check(lib().clb_dcsr_to_coo_deep(controls.handle, C.byref(ac), C.byref(m)))
+    return matrix_coo._from_c(m)
+
+
+def dcsr_to_csr(controls, a: matrix_dcsr) -> matrix_csr:
+    m, ac = _Csr(), a._c()
+    _check(lib().clb_dcsr_to_csr(controls.handle, C.byref(ac), C.byref(m)))
+    return matrix_csr._from_c(m)
+
+
+def csr_to_dcsr(controls, a: matrix_csr) -> matrix_dcsr:
+    m, ac = _Dcsr(), a._c()
+    _check(lib().clb_csr_to_dcsr(controls.handle, C.byref(ac), C.byref(m)))
+    return matrix_dcsr._from_c(m)
+
+
+def prefix_sum(controls, buf: Buffer, n: int) -> int:
+    """prefix_sum (cl_operations.cpp:4-88): in-place exclusive scan, returns the total."""
+    t = C.c_uint32()
+    _check(lib().clb_prefix_sum(controls.handle, _raw(buf), n, C.byref(t)))
+    return t.value
+
+
+def fill_with(controls, buf: Buffer, n: int, value: int):
+    """fill_with (cl_operations.cpp:90-100)."""
+    _check(lib().clb_fill_with(controls.handle, _raw(buf), n, value))
+
+
+def kernel_lookup(program: str, kernel: str):
+    """KernelCache::get_kernel (kernel_cache.hpp:46-56) name check."""
+    _check(lib().clb_kernel_lookup(program.encode(), kernel.encode()))
+
+
+# ---------------------------------------------------------------------------
+def _matrix_multiplication_hash(controls, a: matrix_dcsr, b: matrix_dcsr) -> matrix_dcsr:
+    """dcsr::matrix_multiplication_hash (dcsr_matrix_multiplication_hash.cpp:51-119)."""
+    m, ac, bc = _Dcsr(), a._c(), b._c()
+    _check(lib().clb_dcsr_spgemm_hash(controls.handle, C.byref(ac), C.byref(bc), C.byref(m)))
+    return matrix_dcsr._from_c(m)
+
+
+def _spgemm(controls, a: matrix_dcsr, b: matrix_dcsr, mode=SPGEMM_PLAIN, row_begin=0, row_end=0, timing=False):
+    """The hash SpGEMM with a mode and an optional row block; returns (C, stats)."""
+    m, ac, bc = _Dcsr(), a._c(), b._c()
+    o = SpgemmOpts(mode, row_begin, row_end, int(timing))
+    st = SpgemmStats()
+    _check(lib().clb_dcsr_spgemm(controls.handle, C.byref(ac), C.byref(bc), C.byref(o), C.byref(m), C.byref(st)))
+    return matrix_dcsr._from_c(m), st.as_dict()
+
+
+def _kronecker_product(controls, a: matrix_dcsr, b: matrix_dcsr) -> matrix_dcsr:
+    """dcsr::kronecker_product (dcsr_kronecker_product.cpp:6-65)."""
+    m, ac, bc = _Dcsr(), a._c(), b._c()
+    _check(lib().clb_dcsr_kronecker(controls.handle, C.byref(ac), C.byref(bc), C.byref(m)))
+    return matrix_dcsr._from_c(m)
+
+
+def _dcsr_addition(controls, a: matrix_dcsr, b: matrix_dcsr) -> matrix_dcsr:
+    m, ac, bc = _Dcsr(), a._c(), b._c()
+    _check(lib().clb_dcsr_addition(controls.handle, C.byref(ac), C.byref(bc), C.byref(m)))
+    return matrix_dcsr._from_c(m)
+
+
+def _coo_matrix_addition(controls, a: matrix_coo, b: matrix_coo) -> matrix_coo:
+    """coo::matrix_addition (coo_matrix_addition.cpp:32-73)."""
+    m, ac, bc = _Coo(), a._c(), b._c()
+    _check(lib().clb_coo_addition(controls.handle, C.byref(ac), C.byref(bc), C.byref(m)))
+    return matrix_coo._from_c(m)
+
+
+def _coo_sort_arrays(controls, rows: Buffer, cols: Buffer, n: int):
+    """coo::sort_arrays (coo_initialization.cpp:6-32)."""
+    _check(lib().clb_coo_sort_arrays(controls.handle, _raw(rows), _raw(cols), n))
+
+
+def _csr_matrix_addition(controls, a: matrix_csr, b: matrix_csr) -> matrix_csr:
+    """csr::matrix_addition (csr_addition.cpp:26-168)."""
+    m, ac, bc = _Csr(), a._c(), b._c()
+    _check(lib().clb_csr_addition(controls.handle, C.byref(ac), C.byref(bc), C.byref(m)))
+    return matrix_csr._from_c(m)
+
+
+dcsr = types.SimpleNamespace(matrix_multiplication_hash=_matrix_multiplication_hash,
+                             kronecker_product=_kronecker_product, spgemm=_spgemm, addition=_dcsr_addition)
+coo = types.SimpleNamespace(matrix_addition=_coo_matrix_addition, sort_arrays=_coo_sort_arrays)
+csr = types.SimpleNamespace(matrix_addition=_csr_matrix_addition)

@@ -1,0 +1,150 @@
+"""ctypes declarations of include/clbool_c.h (libclbool_amd.so).
+
+The library is loaded from this package directory only; when it is missing
+the import fails loudly (there is no CPU fallback of any operation).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libclbool_amd.so")
+
+u32p = C.POINTER(C.c_uint32)
+buf_t = C.c_void_p
+ctl_t = C.c_void_p
+
+CLB_MAX_BINS = 32
+SPGEMM_PLAIN = 0
+SPGEMM_LEARNED = 1
+
+
+class Dcsr(C.Structure):
+    _fields_ = [("rpt", buf_t), ("rows", buf_t), ("cols", buf_t),
+                ("nrows", C.c_uint32), ("ncols", C.c_uint32), ("nnz", C.c_uint32), ("nzr", C.c_uint32)]
+
+
+class Coo(C.Structure):
+    _fields_ = [("rows", buf_t), ("cols", buf_t),
+                ("nrows", C.c_uint32), ("ncols", C.c_uint32), ("nnz", C.c_uint32)]
+
+
+class Csr(C.Structure):
+    _fields_ = [("rpt", buf_t), ("cols", buf_t),
+                ("nrows", C.c_uint32), ("ncols", C.c_uint32), ("nnz", C.c_uint32)]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("hash_max_slots", C.c_uint32), ("bitmap_min_products", C.c_uint32),
+                ("bitmap_slot_ratio", C.c_uint32), ("tiny_max_products", C.c_uint32),
+                ("learned_margin_pct", C.c_uint32), ("reserved", C.c_uint32 * 11)]
+
+
+class SpgemmOpts(C.Structure):
+    _fields_ = [("mode", C.c_uint32), ("row_begin", C.c_uint32), ("row_end", C.c_uint32),
+                ("timing", C.c_uint32)]
+
+
+class SpgemmStats(C.Structure):
+    _fields_ = [("products", C.c_uint64), ("nnz", C.c_uint64), ("nzr", C.c_uint32), ("rows", C.c_uint32),
+                ("fallback_rows", C.c_uint32), ("n_bins", C.c_uint32),
+                ("bin_rows", C.c_uint32 * CLB_MAX_BINS), ("bin_products", C.c_uint64 * CLB_MAX_BINS),
+                ("pred_abs_log2_err", C.c_double),
+                ("ms_setup", C.c_double), ("ms_symbolic", C.c_double), ("ms_numeric", C.c_double),
+                ("ms_assemble", C.c_double), ("ms_total", C.c_double)]
+
+    def as_dict(self):
+        d = {f: getattr(self, f) for f, _ in self._fields_ if f not in ("bin_rows", "bin_products")}
+        d["bin_rows"] = list(self.bin_rows)
+        d["bin_products"] = list(self.bin_products)
+        return d
+
+
+class DeviceInfo(C.Structure):
+    _fields_ = [("name", C.c_char * 256), ("arch", C.c_char * 64), ("compute_units", C.c_uint32),
+                ("max_wg_size", C.c_uint32), ("lds_bytes", C.c_uint32), ("global_mem_bytes", C.c_uint64),
+                ("clock_khz", C.c_uint32), ("mem_clock_khz", C.c_uint32), ("mem_bus_width", C.c_uint32)]
+
+
+# (name, restype, argtypes) for every symbol of include/clbool_c.h
+SIGNATURES = [
+    ("clb_last_error", C.c_char_p, []),
+    ("clb_device_count", C.c_int32, [u32p]),
+    ("clb_controls_create", C.c_int32, [C.c_uint32, C.c_uint32, C.POINTER(ctl_t)]),
+    ("clb_controls_destroy", C.c_int32, [ctl_t]),
+    ("clb_device_info_get", C.c_int32, [C.c_uint32, C.POINTER(DeviceInfo)]),
+    ("clb_controls_sync", C.c_int32, [ctl_t]),
+    ("clb_controls_max_wg_size", C.c_int32, [ctl_t, u32p]),
+    ("clb_get_tuning", C.c_int32, [ctl_t, C.POINTER(Tuning)]),
+    ("clb_set_tuning", C.c_int32, [ctl_t, C.POINTER(Tuning)]),
+    ("clb_last_stats", C.c_int32, [ctl_t, C.POINTER(SpgemmStats)]),
+    ("clb_pool_trim", C.c_int32, [ctl_t]),
+    ("clb_buffer_create", C.c_int32, [ctl_t, C.c_uint64, C.POINTER(buf_t)]),
+    ("clb_buffer_from_host", C.c_int32, [ctl_t, C.c_void_p, C.c_uint64, C.POINTER(buf_t)]),
+    ("clb_buffer_read", C.c_int32, [ctl_t, buf_t, C.c_uint64, C.c_uint64, C.c_void_p]),
+    ("clb_buffer_write", C.c_int32, [ctl_t, buf_t, C.c_uint64, C.c_uint64, C.c_void_p]),
+    ("clb_buffer_copy", C.c_int32, [ctl_t, buf_t, C.c_uint64, buf_t, C.c_uint64, C.c_uint64]),
+    ("clb_buffer_retain", C.c_int32, [buf_t]),
+    ("clb_buffer_release", C.c_int32, [buf_t]),
+    ("clb_buffer_device_ptr", C.c_void_p, [buf_t]),
+    ("clb_buffer_size", C.c_uint64, [buf_t]),
+    ("clb_dcsr_release", C.c_int32, [C.POINTER(Dcsr)]),
+    ("clb_coo_release", C.c_int32, [C.POINTER(Coo)]),
+    ("clb_csr_release", C.c_int32, [C.POINTER(Csr)]),
+    ("clb_coo_from_host", C.c_int32, [ctl_t, u32p, u32p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, C.c_int,
+                                      C.POINTER(Coo)]),
+    ("clb_coo_from_buffers", C.c_int32, [ctl_t, C.c_uint32, C.c_uint32, C.c_uint32, buf_t, buf_t, C.c_int,
+                                         C.c_int, C.POINTER(Coo)]),
+    ("clb_coo_sort_arrays", C.c_int32, [ctl_t, buf_t, buf_t, C.c_uint32]),
+    ("clb_dcsr_from_host", C.c_int32, [ctl_t, u32p, u32p, u32p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                       C.POINTER(Dcsr)]),
+    ("clb_csr_from_host", C.c_int32, [ctl_t, u32p, u32p, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(Csr)]),
+    ("clb_coo_to_dcsr_shallow", C.c_int32, [ctl_t, C.POINTER(Coo), C.POINTER(Dcsr)]),
+    ("clb_dcsr_to_coo_shallow", C.c_int32, [ctl_t, C.POINTER(Dcsr), C.POINTER(Coo)]),
+    ("clb_dcsr_to_coo_deep", C.c_int32, [ctl_t, C.POINTER(Dcsr), C.POINTER(Coo)]),
+    ("clb_dcsr_to_csr", C.c_int32, [ctl_t, C.POINTER(Dcsr), C.POINTER(Csr)]),
+    ("clb_csr_to_dcsr", C.c_int32, [ctl_t, C.POINTER(Csr), C.POINTER(Dcsr)]),
+    ("clb_dcsr_spgemm_hash", C.c_int32, [ctl_t, C.POINTER(Dcsr), C.POINTER(Dcsr), C.POINTER(Dcsr)]),
+    ("clb_dcsr_spgemm", C.c_int32, [ctl_t, C.POINTER(Dcsr), C.POINTER(Dcsr), C.POINTER(SpgemmOpts),
+                                    C.POINTER(Dcsr), C.POINTER(SpgemmStats)]),
+    ("clb_dcsr_kronecker", C.c_int32, [ctl_t, C.POINTER(Dcsr), C.POINTER(Dcsr), C.POINTER(Dcsr)]),
+    ("clb_coo_addition", C.c_int32, [ctl_t, C.POINTER(Coo), C.POINTER(Coo), C.POINTER(Coo)]),
+    ("clb_csr_addition", C.c_int32, [ctl_t, C.POINTER(Csr), C.POINTER(Csr), C.POINTER(Csr)]),
+    ("clb_dcsr_addition", C.c_int32, [ctl_t, C.POINTER(Dcsr), C.POINTER(Dcsr), C.POINTER(Dcsr)]),
+    ("clb_prefix_sum", C.c_int32, [ctl_t, buf_t, C.c_uint32, u32p]),
+    ("clb_fill_with", C.c_int32, [ctl_t, buf_t, C.c_uint32, C.c_uint32]),
+    ("clb_kernel_lookup", C.c_int32, [C.c_char_p, C.c_char_p]),
+]
+
+_lib = None
+
+
+def load():
+    """Load libclbool_amd.so with every declared symbol bound."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with `python clbool_amd/build.py` "
+                              "(there is no CPU fallback)")
+        L = C.CDLL(LIB_PATH)
+        for name, res, args in SIGNATURES:
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def declared_symbols(header_path=None):
+    """Function names declared in include/*.h (for the export check)."""
+    import glob
+    import re
+    root = os.path.dirname(HERE)
+    paths = [header_path] if header_path else sorted(glob.glob(os.path.join(root, "include", "*.h")))
+    names = []
+    for p in paths:
+        txt = open(p).read()
+        txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+        names += re.findall(r"^\s*(?:const\s+)?[A-Za-z_][A-Za-z0-9_ ]*?\*?\s*\b(clb_[a-z0-9_]+)\s*\(", txt, flags=re.M)
+    return sorted(set(names))

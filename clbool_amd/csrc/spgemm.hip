@@ -1,0 +1,476 @@
+// spgemm.hip -- host driver of the boolean SpGEMM C = A * B (DCSR).
+//
+// Plain mode follows the two-pass structure of clBool's
+// dcsr::matrix_multiplication_hash (src/dcsr/dcsr_matrix_multiplication_hash.cpp:51-119):
+// count products per row (count_workload, dcsr_matrix_multiplication.cpp:388-404),
+// bin rows (build_groups_and_allocate_hash, hash.cpp:301-345 -- here on the
+// device, no per-row host loop), symbolic pass (count_nnz, :122-201), scan
+// (fill_nnz's prefix_sum, :218), numeric pass (:235-280), drop empty rows
+// (prepare_positions + set_positions, :282-296).  Learned mode lives in
+// learned.hip and shares the kernels.
+#include "spgemm.hpp"
+
+#include <chrono>
+#include <sstream>
+
+namespace clb {
+namespace spg {
+
+__global__ void k_brange_set(const u32 *__restrict__ b_rpt, const u32 *__restrict__ b_rows, u32 b_nzr, u32 mapn,
+                             uint2 *__restrict__ brange) {
+    const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= b_nzr) return;
+    const u32 r = b_rows[i];
+    if (r < mapn) brange[r] = make_uint2(b_rpt[i], b_rpt[i + 1]);
+}
+
+// count_workload (count_workload.cl:30-53): products of each A row; G lanes
+// share a row.
+template <int G>
+__global__ __launch_bounds__(256) void k_count(Mats g, u32 nrows, u32 *__restrict__ prod, u64 *__restrict__ total) {
+    const u32 gid = blockIdx.x * 256 + threadIdx.x;
+    const u32 row = gid / G, sub = gid % G;
+    u32 p = 0;
+    if (row < nrows) {
+        const u32 i = g.row0 + row;
+        const u32 je = g.a_rpt[i + 1];
+        for (u32 j = g.a_rpt[i] + sub; j < je; j += G) {
+            const u32 k = g.a_cols[j];
+            if (k < g.mapn) {
+                const uint2 r = g.brange[k];
+                p += r.y - r.x;
+            }
+        }
+    }
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) p += __shfl_xor(p, o, G);
+    const bool lead = row < nrows && sub == 0;
+    if (lead) prod[row] = p;
+    u64 s = dev::wave_sum<u64>(lead ? (u64)p : 0ull);
+    if (dev::lane() == 0 && s) atomicAdd(reinterpret_cast<unsigned long long *>(total), (unsigned long long)s);
+}
+
+// Histogram of bins.  key == nullptr: key = min(P, ncols) (symbolic sizing).
+// Rows in BIN_EMPTY get nnz_row = 0 when zero_empty is set.
+__global__ __launch_bounds__(256) void k_bin_hist(const u32 *__restrict__ P, const u32 *__restrict__ key, u32 n,
+                                                  BinCfg c, u32 *__restrict__ hist, u64 *__restrict__ binprod,
+                                                  u32 *__restrict__ nnz_row) {
+    __shared__ u32 h[NBINS];
+    __shared__ unsigned long long hp[NBINS];
+    if (threadIdx.x < NBINS) {
+        h[threadIdx.x] = 0;
+        hp[threadIdx.x] = 0;
+    }
+    __syncthreads();
+    const u32 i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) {
+        const u32 p = P[i];
+        const u32 k = key ? key[i] : (p < c.ncols ? p : c.ncols);
+        const u32 b = classify(p, k, c);
+        atomicAdd(&h[b], 1u);
+        atomicAdd(&hp[b], (unsigned long long)p);
+        if (b == BIN_EMPTY && nnz_row) nnz_row[i] = 0;
+    }
+    __syncthreads();
+    if (threadIdx.x < NBINS && h[threadIdx.x]) {
+        atomicAdd(&hist[threadIdx.x], h[threadIdx.x]);
+        atomicAdd(reinterpret_cast<unsigned long long *>(&binprod[threadIdx.x]), hp[threadIdx.x]);
+    }
+}
+
+// Exclusive scan of the bin histogram into cursor[0..NBINS] (one wave).
+__global__ void k_bin_offsets(const u32 *__restrict__ hist, u32 *__restrict__ cursor) {
+    const int l = dev::lane();
+    const u32 v = l < (int)NBINS ? hist[l] : 0u;
+    const u32 inc = dev::wave_incl_scan(v);
+    if (l < (int)NBINS) cursor[l] = inc - v;
+    if (l == (int)NBINS - 1) cursor[NBINS] = inc;
+}
+
+// Rows grouped by bin (order inside a bin is not significant: output
+// positions come from the scan of row sizes, not from the processing order).
+__global__ __launch_bounds__(256) void k_bin_scatter(const u32 *__restrict__ P, const u32 *__restrict__ key, u32 n,
+                                                     BinCfg c, u32 *__restrict__ cursor, u32 *__restrict__ perm) {
+    __shared__ u32 h[NBINS];
+    __shared__ u32 base[NBINS];
+    if (threadIdx.x < NBINS) h[threadIdx.x] = 0;
+    __syncthreads();
+    const u32 i = blockIdx.x * 256 + threadIdx.x;
+    u32 b = BIN_EMPTY, local = 0;
+    if (i < n) {
+        const u32 p = P[i];
+        const u32 k = key ? key[i] : (p < c.ncols ? p : c.ncols);
+        b = classify(p, k, c);
+        if (b != BIN_EMPTY) local = atomicAdd(&h[b], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x < NBINS && h[threadIdx.x]) base[threadIdx.x] = atomicAdd(&cursor[threadIdx.x], h[threadIdx.x]);
+    __syncthreads();
+    if (i < n && b != BIN_EMPTY) perm[base[b] + local] = i;
+}
+
+__global__ void k_nonzero_flags(const u32 *__restrict__ nnz_row, u32 n, u32 *__restrict__ flags) {
+    const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) flags[i] = nnz_row[i] != 0;
+}
+
+// prepare_for_shift_empty_rows + set_positions_pointers_and_rows
+// (prepare_positions.cl:69-84, set_positions.cl:46-66).
+__global__ void k_set_positions(const u32 *__restrict__ nnz_row, const u64 *__restrict__ off,
+                                const u32 *__restrict__ pos, u32 n, const u32 *__restrict__ a_rows, u32 row0,
+                                u32 *__restrict__ c_rpt, u32 *__restrict__ c_rows, u32 nnz, u32 nzr) {
+    const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) c_rpt[nzr] = nnz;
+    if (i >= n || nnz_row[i] == 0) return;
+    const u32 q = pos[i];
+    c_rpt[q] = static_cast<u32>(off[i]);
+    c_rows[q] = a_rows[row0 + i];
+}
+
+// ---------------------------------------------------------------------------
+BinCfg make_bincfg(const clb_controls_s *ctl, u32 ncols) {
+    BinCfg c{};
+    c.ncols = ncols;
+    c.tiny_max = ctl->tuning.tiny_max_products;
+    u32 hb = 0;
+    while ((1u << (hb + 1)) <= ctl->tuning.hash_max_slots) ++hb;
+    c.hash_max_bits = hb < HASH_MIN_BITS ? HASH_MIN_BITS : (hb > HASH_MAX_BITS ? HASH_MAX_BITS : hb);
+    c.bitmap_min = ctl->tuning.bitmap_min_products;
+    c.bitmap_ratio = ctl->tuning.bitmap_slot_ratio;
+    const u64 words = ((u64)ncols + 31) / 32;
+    u32 wb = 6;
+    while (wb < 15 && (1ull << wb) < words) ++wb;
+    c.bm_bits = wb;
+    return c;
+}
+
+void prepare(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &b, u32 row_begin, u32 nrows, Setup &s) {
+    s.nrows = nrows;
+    s.row0 = row_begin;
+    s.mapn = b.nrows;
+    s.brange.reset(ctl, s.mapn ? s.mapn : 1);
+    fill_u64(ctl, reinterpret_cast<u64 *>(s.brange.ptr), s.mapn, 0ull);
+    if (b.nzr) {
+        k_brange_set<<<div_up(b.nzr, 256), 256, 0, ctl->stream>>>(u32p(b.rpt), u32p(b.rows), b.nzr, s.mapn,
+                                                                   s.brange.ptr);
+        CLB_LAUNCHED("brange_set");
+    }
+    s.g.a_rpt = u32p(a.rpt);
+    s.g.a_rows = u32p(a.rows);
+    s.g.a_cols = u32p(a.cols);
+    s.g.brange = s.brange.ptr;
+    s.g.b_cols = u32p(b.cols);
+    s.g.mapn = s.mapn;
+    s.g.row0 = row_begin;
+    s.g.ncols = b.ncols;
+    s.cfg = make_bincfg(ctl, b.ncols);
+    s.prod.reset(ctl, nrows);
+    s.scal.reset(ctl, 8);  // [0] total products, [1] nnz total, [2] nzr total, [3] overflow flag
+    fill_u64(ctl, s.scal.ptr, 8, 0ull);
+    // lanes per row from the mean row length of the block
+    const u64 nnz_blk = 0;  // unknown without a readback; use the matrix mean
+    (void)nnz_blk;
+    const double mean = a.nzr ? (double)a.nnz / a.nzr : 1.0;
+    int G = 1;
+    while (G < 64 && G * 2 <= mean) G *= 2;
+    const u32 grid = div_up((u64)nrows * G, 256);
+    switch (G) {
+#define CNT(GG) \
+    case GG: k_count<GG><<<grid, 256, 0, ctl->stream>>>(s.g, nrows, s.prod.ptr, s.scal.ptr); break;
+        CNT(1) CNT(2) CNT(4) CNT(8) CNT(16) CNT(32) CNT(64)
+#undef CNT
+    }
+    CLB_LAUNCHED("count_workload");
+}
+
+void bin_rows(clb_controls_s *ctl, const Setup &s, const u32 *key, u32 *nnz_row_zero, DevArray<u32> &perm,
+              DevArray<u32> &binbuf, u32 hist_host[NBINS + 1], u64 binprod_host[NBINS], bool readback_now) {
+    // binbuf: hist[NBINS] | cursor[NBINS+1] | (pad) | binprod as u64[NBINS]
+    binbuf.reset(ctl, 2 * NBINS + 2 + 2 * NBINS);
+    fill_u32(ctl, binbuf.ptr, binbuf.n, 0u);
+    u32 *hist = binbuf.ptr;
+    u32 *cursor = binbuf.ptr + NBINS;
+    u64 *binprod = reinterpret_cast<u64 *>(binbuf.ptr + 2 * NBINS + 2);
+    perm.reset(ctl, s.nrows ? s.nrows : 1);
+    const u32 grid = div_up(s.nrows, 256);
+    k_bin_hist<<<grid, 256, 0, ctl->stream>>>(s.prod.ptr, key, s.nrows, s.cfg, hist, binprod, nnz_row_zero);
+    CLB_LAUNCHED("bin_hist");
+    k_bin_offsets<<<1, 64, 0, ctl->stream>>>(hist, cursor);
+    CLB_LAUNCHED("bin_offsets");
+    // scatter advances the cursor; keep the offsets: recompute from hist on host
+    k_bin_scatter<<<grid, 256, 0, ctl->stream>>>(s.prod.ptr, key, s.nrows, s.cfg, cursor, perm.ptr);
+    CLB_LAUNCHED("bin_scatter");
+    if (readback_now) {
+        u32 h[NBINS];
+        readback(ctl, h, hist, sizeof(h));
+        hist_host[0] = 0;
+        for (u32 b = 0; b < NBINS; ++b) hist_host[b + 1] = hist_host[b] + h[b];
+        if (binprod_host) readback(ctl, binprod_host, binprod, NBINS * sizeof(u64));
+    }
+}
+
+// Launch the per-bin kernels of one pass.  off[b]..off[b+1] is bin b's range
+// of perm.  numeric == false: write nnz_row; true: write columns.
+void launch_bins(clb_controls_s *ctl, const Setup &s, const u32 *perm, const u32 off[NBINS + 1], bool numeric,
+                 u32 *nnz_row, const u64 *out_off, u32 *out_cols, u32 *overflow) {
+    fork_side(ctl);
+    int k = 0;
+    auto next = [&]() { return side_stream(ctl, k++); };
+    // TINY
+    {
+        const u32 n = off[BIN_TINY + 1] - off[BIN_TINY];
+        if (n) {
+            k_tiny<4><<<div_up(n, 4), 256, 0, next()>>>(s.g, perm + off[BIN_TINY], n, nnz_row, out_off,
+                                                          numeric ? out_cols : nullptr);
+            CLB_LAUNCHED("tiny");
+        }
+    }
+    // HASH
+    for (u32 bits = HASH_MIN_BITS; bits <= HASH_MAX_BITS; ++bits) {
+        const u32 b = BIN_HASH0 + (bits - HASH_MIN_BITS);
+        const u32 n = off[b + 1] - off[b];
+        if (!n) continue;
+        const u32 *pp = perm + off[b];
+        hipStream_t st = next();
+#define HK(BITS, WG)                                                                                          \
+    case BITS:                                                                                                \
+        if (numeric)                                                                                          \
+            k_hash<BITS, WG, true><<<n, WG, 0, st>>>(s.g, pp, nnz_row, out_off, out_cols, overflow);          \
+        else                                                                                                  \
+            k_hash<BITS, WG, false><<<n, WG, 0, st>>>(s.g, pp, nnz_row, out_off, out_cols, overflow);         \
+        break;
+        switch (bits) {
+            HK(7, 64) HK(8, 64) HK(9, 64) HK(10, 128) HK(11, 128) HK(12, 256) HK(13, 512) HK(14, 1024)
+        }
+#undef HK
+        CLB_LAUNCHED("hash");
+    }
+    // BITMAP (all classes in one launch, largest rows first)
+    {
+        const u32 n = off[NBINS] - off[BIN_BITMAP0];
+        if (n) {
+            const u32 *pp = perm + off[BIN_BITMAP0];
+            hipStream_t st = next();
+#define BK(WB, WG)                                                                                            \
+    case WB:                                                                                                  \
+        if (numeric)                                                                                          \
+            k_bitmap<WB, WG, true><<<n, WG, 0, st>>>(s.g, pp, nnz_row, out_off, out_cols);                    \
+        else                                                                                                  \
+            k_bitmap<WB, WG, false><<<n, WG, 0, st>>>(s.g, pp, nnz_row, out_off, out_cols);                   \
+        break;
+            switch (s.cfg.bm_bits) {
+                BK(6, 64) BK(7, 64) BK(8, 64) BK(9, 256) BK(10, 256) BK(11, 256) BK(12, 512) BK(13, 512)
+                BK(14, 1024) BK(15, 1024)
+            }
+#undef BK
+            CLB_LAUNCHED("bitmap");
+        }
+    }
+    join_side(ctl);
+}
+
+struct Timer {
+    bool on;
+    hipStream_t st;
+    std::vector<hipEvent_t> ev;
+    Timer(bool enable, hipStream_t s) : on(enable), st(s) {}
+    void mark() {
+        if (!on) return;
+        hipEvent_t e;
+        CLB_HIP(hipEventCreate(&e), EVENT_WAITING_ERROR);
+        CLB_HIP(hipEventRecord(e, st), EVENT_WAITING_ERROR);
+        ev.push_back(e);
+    }
+    double ms(size_t i, size_t j) const {
+        if (!on || j >= ev.size()) return 0;
+        float m = 0;
+        (void)hipEventElapsedTime(&m, ev[i], ev[j]);
+        return m;
+    }
+    ~Timer() {
+        for (auto e : ev) (void)hipEventDestroy(e);
+    }
+};
+
+void assemble(clb_controls_s *ctl, const Setup &s, const clb_dcsr &a, const u32 *nnz_row, const u64 *off,
+              const u32 *pos, u64 nnz, u32 nzr, clb_buffer_s *cols, clb_dcsr &out) {
+    out.cols = cols;
+    out.rpt = buffer_new(ctl, 4ull * (nzr + 1));
+    out.rows = buffer_new(ctl, 4ull * (nzr ? nzr : 1));
+    k_set_positions<<<div_up(s.nrows ? s.nrows : 1, 256), 256, 0, ctl->stream>>>(
+        nnz_row, off, pos, s.nrows, s.g.a_rows, s.row0, u32p(out.rpt), u32p(out.rows), (u32)nnz, nzr);
+    CLB_LAUNCHED("set_positions");
+    out.nnz = (u32)nnz;
+    out.nzr = nzr;
+}
+
+// Offsets and empty-row compaction of the exact row sizes.  Results land in
+// s.scal[1] (nnz) and s.scal[2] (nzr).
+void scan_sizes(clb_controls_s *ctl, Setup &s, const u32 *nnz_row, DevArray<u64> &off, DevArray<u32> &pos) {
+    off.reset(ctl, s.nrows + 1);
+    pos.reset(ctl, s.nrows + 1);
+    exclusive_scan_u32_to_u64(ctl, nnz_row, off.ptr, s.nrows, s.scal.ptr + 1);
+    k_nonzero_flags<<<div_up(s.nrows, 256), 256, 0, ctl->stream>>>(nnz_row, s.nrows, pos.ptr);
+    CLB_LAUNCHED("nonzero_flags");
+    exclusive_scan_u32(ctl, pos.ptr, pos.ptr, s.nrows, reinterpret_cast<u32 *>(s.scal.ptr + 2));
+}
+
+void spgemm_plain(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &b, u32 row_begin, u32 row_end,
+                  bool timing, clb_dcsr &out, clb_spgemm_stats &st) {
+    Timer tm(timing, ctl->stream);
+    tm.mark();  // 0
+    const u32 nrows = row_end - row_begin;
+    Setup s;
+    prepare(ctl, a, b, row_begin, nrows, s);
+    DevArray<u32> perm, binbuf;
+    u32 off[NBINS + 1];
+    u64 bprod[NBINS];
+    DevArray<u32> nnz_row(ctl, nrows);
+    bin_rows(ctl, s, nullptr, nnz_row.ptr, perm, binbuf, off, bprod, false);
+    u64 total_products = 0;
+    {
+        u32 h[NBINS];
+        readback(ctl, h, binbuf.ptr, sizeof(h));
+        off[0] = 0;
+        for (u32 k = 0; k < NBINS; ++k) off[k + 1] = off[k] + h[k];
+        readback(ctl, bprod, binbuf.ptr + 2 * NBINS + 2, sizeof(bprod));
+        readback(ctl, &total_products, s.scal.ptr, 8);
+    }
+    st.products = total_products;
+    st.rows = nrows;
+    st.n_bins = NBINS;
+    for (u32 k = 0; k < NBINS; ++k) {
+        st.bin_rows[k] = off[k + 1] - off[k];
+        st.bin_products[k] = bprod[k];
+    }
+    tm.mark();  // 1
+    if (total_products == 0) {
+        out.nrows = a.nrows;
+        out.ncols = b.ncols;
+        return;
+    }
+    launch_bins(ctl, s, perm.ptr, off, false, nnz_row.ptr, nullptr, nullptr,
+                reinterpret_cast<u32 *>(s.scal.ptr + 3));
+    tm.mark();  // 2
+    DevArray<u64> offs;
+    DevArray<u32> pos;
+    scan_sizes(ctl, s, nnz_row.ptr, offs, pos);
+    DevArray<u32> perm2, binbuf2;
+    u32 off2[NBINS + 1];
+    bin_rows(ctl, s, nnz_row.ptr, nullptr, perm2, binbuf2, off2, nullptr, false);
+    u64 sc[4];
+    {
+        u32 h[NBINS];
+        readback(ctl, h, binbuf2.ptr, sizeof(h));
+        off2[0] = 0;
+        for (u32 k = 0; k < NBINS; ++k) off2[k + 1] = off2[k] + h[k];
+        readback(ctl, sc, s.scal.ptr, sizeof(sc));
+    }
+    tm.mark();  // 3
+    if (sc[3]) raise(RUN_KERNEL_FAILURE, "hash table overflow in the symbolic pass");
+    const u64 nnz = sc[1];
+    const u32 nzr = static_cast<u32>(sc[2]);
+    if (nnz > 0xFFFFFFFFull) {
+        std::ostringstream os;
+        os << "result has " << nnz << " nonzeros, more than uint32 indices allow; multiply by row blocks";
+        raise(INVALID_VALUE, os.str());
+    }
+    out.nrows = a.nrows;
+    out.ncols = b.ncols;
+    if (nnz == 0) return;
+    clb_buffer_s *cols = buffer_new(ctl, 4ull * nnz);
+    try {
+        launch_bins(ctl, s, perm2.ptr, off2, true, nnz_row.ptr, offs.ptr, u32p(cols),
+                    reinterpret_cast<u32 *>(s.scal.ptr + 3));
+        tm.mark();  // 4
+        assemble(ctl, s, a, nnz_row.ptr, offs.ptr, pos.ptr, nnz, nzr, cols, out);
+        tm.mark();  // 5
+        u64 flag = 0;
+        readback(ctl, &flag, s.scal.ptr + 3, 8);
+        if (flag) raise(RUN_KERNEL_FAILURE, "numeric pass disagreed with the symbolic row sizes");
+    } catch (...) {
+        (void)hipStreamSynchronize(ctl->stream);
+        clb_dcsr_release(&out);
+        if (!out.cols) buffer_release(cols);
+        throw;
+    }
+    st.nnz = nnz;
+    st.nzr = nzr;
+    if (timing) {
+        st.ms_setup = tm.ms(0, 1);
+        st.ms_symbolic = tm.ms(1, 2);
+        st.ms_numeric = tm.ms(3, 4);
+        st.ms_assemble = tm.ms(2, 3) + tm.ms(4, 5);
+        st.ms_total = tm.ms(0, 5);
+    }
+}
+
+void check_dims(const clb_dcsr &a, const clb_dcsr &b) {
+    // hash.cpp:56-61
+    if (a.ncols != b.nrows) {
+        std::ostringstream s;
+        s << "Invalid input matrix size! a : " << a.nrows << " x " << a.ncols << ", b: " << b.nrows << " x "
+          << b.ncols;
+        raise(INVALID_ARGUMENT, s.str());
+    }
+}
+
+void spgemm_dispatch(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &b, const clb_spgemm_opts &o,
+                     clb_dcsr &out, clb_spgemm_stats &st) {
+    std::memset(&st, 0, sizeof(st));
+    check_dims(a, b);
+    clb_dcsr r{};
+    r.nrows = a.nrows;
+    r.ncols = b.ncols;
+    u32 rb = o.row_begin, re = o.row_end ? o.row_end : a.nzr;
+    if (re > a.nzr || rb > re) raise(INVALID_ARGUMENT, "row block outside the stored rows of A");
+    // hash.cpp:64-67: an empty operand gives an empty result
+    if (a.nnz == 0 || b.nnz == 0 || rb == re) {
+        out = r;
+        return;
+    }
+    auto t0 = std::chrono::steady_clock::now();
+    if (o.mode == CLB_SPGEMM_LEARNED)
+        spgemm_learned(ctl, a, b, rb, re, o.timing != 0, r, st);
+    else if (o.mode == CLB_SPGEMM_PLAIN)
+        spgemm_plain(ctl, a, b, rb, re, o.timing != 0, r, st);
+    else
+        raise(INVALID_ARGUMENT, "unknown SpGEMM mode");
+    if (!o.timing)
+        st.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    ctl->last = st;
+    out = r;
+}
+
+}  // namespace spg
+}  // namespace clb
+
+using namespace clb;
+
+extern "C" {
+
+clb_status clb_dcsr_spgemm_hash(clb_controls ctl, const clb_dcsr *a, const clb_dcsr *b, clb_dcsr *out) {
+    return run_guarded([&] {
+        clb_spgemm_opts o{};
+        o.mode = CLB_SPGEMM_PLAIN;
+        clb_spgemm_stats st;
+        clb_dcsr r{};
+        spg::spgemm_dispatch(ctl, *a, *b, o, r, st);
+        *out = r;
+    });
+}
+
+clb_status clb_dcsr_spgemm(clb_controls ctl, const clb_dcsr *a, const clb_dcsr *b, const clb_spgemm_opts *opts,
+                           clb_dcsr *out, clb_spgemm_stats *stats) {
+    return run_guarded([&] {
+        clb_spgemm_opts o{};
+        if (opts) o = *opts;
+        clb_spgemm_stats st;
+        clb_dcsr r{};
+        spg::spgemm_dispatch(ctl, *a, *b, o, r, st);
+        *out = r;
+        if (stats) *stats = st;
+    });
+}
+
+}  // extern "C"

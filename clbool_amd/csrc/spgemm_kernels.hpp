@@ -1,0 +1,346 @@
+// spgemm_kernels.hpp -- device side of the boolean hash SpGEMM.
+//
+// Restates the behaviour of clBool's dcsr::matrix_multiplication_hash
+// (src/dcsr/dcsr_matrix_multiplication_hash.cpp:51-119 with the kernels of
+// src/cl/count_workload.cl and src/cl/hash/*.cl) for CDNA4:
+//
+//  * B rows are found through a dense row-range map (one 8-byte load per A
+//    nonzero) instead of a binary search over b_rows (count_workload.cl:8-27).
+//  * Rows are binned by their intermediate-product count P (and, in the
+//    numeric pass, by their exact output size N) as in hash.cpp:27-36, but
+//    the bins are: TINY (one wave per row, register bitonic sort), HASH with
+//    LDS tables of 2^7..2^14 slots (compact, then LDS bitonic sort of the
+//    N live entries only), and BITMAP (an LDS bitmap over a column window;
+//    the set bits come out in column order, so nothing is sorted).  The
+//    reference's single-workgroup global-memory table plus global bitonic
+//    sort for long rows (hash_global.cl:104-188) has no counterpart.
+//  * Products of a row are enumerated load-balanced: B rows of >= 256
+//    entries by the whole workgroup, of >= 64 by one wave, shorter ones by
+//    lane-level expansion over a wave prefix sum.
+#pragma once
+
+#include "clb_internal.hpp"
+#include "device_utils.hpp"
+
+namespace clb {
+namespace spg {
+
+// Bin ids.
+constexpr u32 NBINS = 32;
+constexpr u32 BIN_EMPTY = 0;
+constexpr u32 BIN_TINY = 1;
+constexpr u32 BIN_HASH0 = 2;     // 2 + (log2 slots - 7)
+constexpr u32 HASH_MIN_BITS = 7;
+constexpr u32 HASH_MAX_BITS = 14;
+constexpr u32 BIN_BITMAP0 = 10;  // 10 + (31 - floor(log2 P)), clamped to 31
+constexpr u32 BIG_ROW = 256;     // B rows at least this long are shared by the workgroup
+constexpr u32 LONG_ROW = 64;     // B rows at least this long are shared by a wave
+
+struct Mats {
+    const u32 *a_rpt;
+    const u32 *a_rows;
+    const u32 *a_cols;
+    const uint2 *brange;  // B row id -> [begin, end) into b_cols
+    const u32 *b_cols;
+    u32 mapn;             // entries of brange
+    u32 row0;             // first A stored row of the block
+    u32 ncols;            // columns of B (and C)
+};
+
+struct BinCfg {
+    u32 ncols;
+    u32 tiny_max;
+    u32 hash_max_bits;
+    u32 bitmap_min;
+    u32 bitmap_ratio;
+    u32 bm_bits;  // log2 of the bitmap window in 32-bit words
+};
+
+__device__ __forceinline__ u32 ceil_log2(u32 x) { return x <= 1 ? 0 : 32 - __clz(x - 1); }
+
+// P: products of the row; key: the number of distinct columns the table
+// must hold (min(P, ncols) before the symbolic pass, N after it).
+__device__ __forceinline__ u32 classify(u32 P, u32 key, const BinCfg &c) {
+    if (P == 0 || key == 0) return BIN_EMPTY;
+    if (P <= c.tiny_max) return BIN_TINY;
+    u32 bits = ceil_log2(2 * key);
+    if (bits < HASH_MIN_BITS) bits = HASH_MIN_BITS;
+    const bool bitmap = P >= c.bitmap_min || bits > c.hash_max_bits ||
+                        (1ull << c.bm_bits) <= (u64)c.bitmap_ratio * (1ull << bits);
+    if (!bitmap) return BIN_HASH0 + (bits - HASH_MIN_BITS);
+    const u32 lp = 31 - __clz(P);
+    const u32 b = BIN_BITMAP0 + (31 - lp);
+    return b < NBINS ? b : NBINS - 1;
+}
+
+// ---------------------------------------------------------------------------
+// Product enumeration of one A row [js, je) by a workgroup of WG threads.
+// `big` is LDS scratch of 2*WG+1 words (unused when WG == 64).  f(col) is
+// called once per product, possibly from divergent lanes.
+template <int WG, typename F>
+__device__ __forceinline__ void for_each_product(const Mats &g, u32 js, u32 je, u32 *big, F &&f) {
+    const int l = dev::lane();
+    const int w = dev::wave_id();
+    constexpr int NW = WG / dev::WAVE;
+    for (u32 base = js; base < je; base += WG) {
+        const u32 j = base + threadIdx.x;
+        u32 bx = 0, len = 0;
+        if (j < je) {
+            const u32 k = g.a_cols[j];
+            if (k < g.mapn) {
+                const uint2 r = g.brange[k];
+                bx = r.x;
+                len = r.y - r.x;
+            }
+        }
+        if (NW > 1) {
+            if (threadIdx.x == 0) big[2 * WG] = 0;
+            __syncthreads();
+            if (len >= BIG_ROW) {
+                const u32 s = atomicAdd(&big[2 * WG], 1u);
+                big[2 * s] = bx;
+                big[2 * s + 1] = len;
+                len = 0;
+            }
+        }
+        // wave-shared long rows
+        unsigned long long lm = __ballot(len >= LONG_ROW);
+        while (lm) {
+            const int s = __ffsll(lm) - 1;
+            lm &= lm - 1;
+            const u32 sb = __shfl(bx, s, dev::WAVE);
+            const u32 sl = __shfl(len, s, dev::WAVE);
+            for (u32 q = l; q < sl; q += dev::WAVE) f(g.b_cols[sb + q]);
+        }
+        // lane expansion of short rows
+        const u32 sl = len < LONG_ROW ? len : 0;
+        const u32 inc = dev::wave_incl_scan(sl);
+        const u32 tot = __shfl(inc, dev::WAVE - 1, dev::WAVE);
+        const u32 exc = inc - sl;
+        for (u32 p0 = 0; p0 < tot; p0 += dev::WAVE) {
+            const u32 p = p0 + l;
+            int s = 0;
+            u32 es = 0;
+#pragma unroll
+            for (int step = 32; step > 0; step >>= 1) {
+                const u32 e = __shfl(exc, s + step, dev::WAVE);
+                if (e <= p) {
+                    s += step;
+                    es = e;
+                }
+            }
+            const u32 sb = __shfl(bx, s, dev::WAVE);
+            if (p < tot) f(g.b_cols[sb + (p - es)]);
+        }
+        if (NW > 1) {
+            __syncthreads();
+            const u32 nb = big[2 * WG];
+            for (u32 e = 0; e < nb; ++e) {
+                const u32 sb = big[2 * e], sl2 = big[2 * e + 1];
+                for (u32 q = threadIdx.x; q < sl2; q += WG) f(g.b_cols[sb + q]);
+            }
+            __syncthreads();
+        }
+        (void)w;
+    }
+}
+
+// Insert into an LDS open-addressing table of 2^BITS slots (linear probing,
+// EMPTY marks a free slot).  Returns 1 if col was new, 0 if present, and 2
+// if the table is full (only possible when a capacity guess was too small).
+template <int BITS>
+__device__ __forceinline__ u32 hash_insert(u32 *table, u32 col) {
+    constexpr u32 T = 1u << BITS;
+    u32 h = dev::hash_slot(col, BITS);
+    for (u32 probe = 0; probe < T; ++probe) {
+        u32 cur = table[h];
+        if (cur == EMPTY) {
+            cur = atomicCAS(&table[h], EMPTY, col);
+            if (cur == EMPTY) return 1;
+        }
+        if (cur == col) return 0;
+        h = (h + 1) & (T - 1);
+    }
+    return 2;
+}
+
+// ---------------------------------------------------------------------------
+// TINY rows (P <= 64): one wave per row, one product per lane, register
+// bitonic sort across the wave, duplicates removed by a neighbour compare.
+// Symbolic (out_cols == nullptr) writes nnz_row; numeric writes the columns.
+template <int WAVES>
+__global__ __launch_bounds__(WAVES * 64) void k_tiny(Mats g, const u32 *__restrict__ perm, u32 count,
+                                                     u32 *__restrict__ nnz_row, const u64 *__restrict__ out_off,
+                                                     u32 *__restrict__ out_cols) {
+    const u32 idx = blockIdx.x * WAVES + dev::wave_id();
+    if (idx >= count) return;
+    const int l = dev::lane();
+    const u32 r = perm[idx];
+    const u32 i = g.row0 + r;
+    const u32 js = g.a_rpt[i], je = g.a_rpt[i + 1];
+    u32 val = EMPTY;
+    u32 run = 0;
+    for (u32 base = js; base < je && run < 64; base += 64) {
+        const u32 j = base + l;
+        u32 bx = 0, len = 0;
+        if (j < je) {
+            const u32 k = g.a_cols[j];
+            if (k < g.mapn) {
+                const uint2 rr = g.brange[k];
+                bx = rr.x;
+                len = rr.y - rr.x;
+            }
+        }
+        const u32 inc = dev::wave_incl_scan(len);
+        const u32 tot = __shfl(inc, 63, 64);
+        const u32 exc = inc - len;
+        const u32 p = (u32)l - run;  // wraps when l < run
+        int s = 0;
+        u32 es = 0;
+#pragma unroll
+        for (int step = 32; step > 0; step >>= 1) {
+            const u32 e = __shfl(exc, s + step, 64);
+            if (e <= p) {
+                s += step;
+                es = e;
+            }
+        }
+        const u32 sb = __shfl(bx, s, 64);
+        if ((u32)l >= run && p < tot) val = g.b_cols[sb + (p - es)];
+        run += tot;
+    }
+    // bitonic sort of 64 values across the wave
+#pragma unroll
+    for (int k = 2; k <= 64; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            const u32 o = __shfl_xor(val, j, 64);
+            const bool up = (l & k) == 0;
+            const bool lower = (l & j) == 0;
+            const u32 mn = o < val ? o : val, mx = o < val ? val : o;
+            val = (lower == up) ? mn : mx;
+        }
+    }
+    const u32 prev = __shfl_up(val, 1, 64);
+    const bool keep = val != EMPTY && (l == 0 || prev != val);
+    const unsigned long long km = __ballot(keep);
+    if (out_cols == nullptr) {
+        if (l == 0) nnz_row[r] = (u32)__popcll(km);
+    } else {
+        const u32 pos = (u32)__popcll(km & ((1ull << l) - 1ull));
+        if (keep && pos < nnz_row[r]) out_cols[out_off[r] + pos] = val;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// HASH rows: LDS table of 2^BITS slots per workgroup.
+template <int BITS, int WG, bool NUMERIC>
+__global__ __launch_bounds__(WG) void k_hash(Mats g, const u32 *__restrict__ perm, u32 *__restrict__ nnz_row,
+                                             const u64 *__restrict__ out_off, u32 *__restrict__ out_cols,
+                                             u32 *__restrict__ overflow) {
+    constexpr u32 T = 1u << BITS;
+    __shared__ u32 table[T];
+    __shared__ u32 big[2 * WG + 1];
+    __shared__ u32 red[WG / 64 + 1];
+    const u32 r = perm[blockIdx.x];
+    const u32 i = g.row0 + r;
+    const u32 js = g.a_rpt[i], je = g.a_rpt[i + 1];
+    for (u32 t = threadIdx.x; t < T; t += WG) table[t] = EMPTY;
+    __syncthreads();
+    u32 mine = 0, full = 0;
+    for_each_product<WG>(g, js, je, big, [&](u32 col) {
+        const u32 rc = hash_insert<BITS>(table, col);
+        mine += rc == 1;
+        full |= rc == 2;
+    });
+    __syncthreads();
+    if (full) atomicOr(overflow, 1u);
+    if (!NUMERIC) {
+        const u32 n = dev::block_sum<u32, WG>(mine, red);
+        if (threadIdx.x == 0) nnz_row[r] = n;
+        return;
+    }
+    // compact the live slots to the front of the table
+    constexpr u32 PER = T / WG > 0 ? T / WG : 1;
+    u32 v[PER];
+    u32 c = 0;
+#pragma unroll
+    for (u32 q = 0; q < PER; ++q) {
+        const u32 t = threadIdx.x * PER + q;
+        v[q] = t < T ? table[t] : EMPTY;
+        c += v[q] != EMPTY;
+    }
+    u32 n;
+    u32 pos = dev::block_excl_scan<u32, WG>(c, red, n);
+#pragma unroll
+    for (u32 q = 0; q < PER; ++q)
+        if (v[q] != EMPTY) table[pos++] = v[q];
+    const u32 n2 = n <= 1 ? 1 : (1u << (32 - __clz(n - 1)));
+    for (u32 t = n + threadIdx.x; t < n2; t += WG) table[t] = EMPTY;
+    __syncthreads();
+    dev::lds_bitonic_sort<WG>(table, n2);
+    // never write past the row's slot, even if the passes disagreed
+    const u32 lim = n < nnz_row[r] ? n : nnz_row[r];
+    if (n != nnz_row[r] && threadIdx.x == 0) atomicOr(overflow, 2u);
+    u32 *out = out_cols + out_off[r];
+    for (u32 t = threadIdx.x; t < lim; t += WG) out[t] = table[t];
+}
+
+// ---------------------------------------------------------------------------
+// BITMAP rows: an LDS bitmap of 2^WBITS words covers 32 * 2^WBITS columns;
+// wider column ranges are processed in slices.
+template <int WBITS, int WG, bool NUMERIC>
+__global__ __launch_bounds__(WG) void k_bitmap(Mats g, const u32 *__restrict__ perm, u32 *__restrict__ nnz_row,
+                                               const u64 *__restrict__ out_off, u32 *__restrict__ out_cols) {
+    constexpr u32 W = 1u << WBITS;
+    constexpr u64 SPAN = 32ull * W;
+    __shared__ u32 bm[W];
+    __shared__ u32 big[2 * WG + 1];
+    __shared__ u32 red[WG / 64 + 1];
+    const u32 r = perm[blockIdx.x];
+    const u32 i = g.row0 + r;
+    const u32 js = g.a_rpt[i], je = g.a_rpt[i + 1];
+    const u32 nslices = (u32)(((u64)g.ncols + SPAN - 1) / SPAN);
+    u32 mine = 0;
+    u64 written = 0;
+    u32 *out = NUMERIC ? out_cols + out_off[r] : nullptr;
+    const u64 lim = NUMERIC ? nnz_row[r] : 0;
+    for (u32 s = 0; s < nslices; ++s) {
+        const u64 base = (u64)s * SPAN;
+        for (u32 t = threadIdx.x; t < W; t += WG) bm[t] = 0;
+        __syncthreads();
+        for_each_product<WG>(g, js, je, big, [&](u32 col) {
+            const u64 d = (u64)col - base;
+            if (d < SPAN) atomicOr(&bm[(u32)(d >> 5)], 1u << (col & 31));
+        });
+        __syncthreads();
+        if (!NUMERIC) {
+            for (u32 t = threadIdx.x; t < W; t += WG) mine += __popc(bm[t]);
+        } else {
+            // ordered emission, WG words per round
+            for (u32 r0 = 0; r0 < W; r0 += WG) {
+                const u32 t = r0 + threadIdx.x;
+                u32 word = bm[t];
+                u32 tot;
+                const u32 p = dev::block_excl_scan<u32, WG>((u32)__popc(word), red, tot);
+                u64 q = written + p;
+                const u32 colbase = (u32)(base + 32ull * t);
+                while (word) {
+                    const u32 b = __ffs(word) - 1;
+                    if (q < lim) out[q] = colbase + b;
+                    ++q;
+                    word &= word - 1;
+                }
+                written += tot;
+            }
+        }
+        __syncthreads();
+    }
+    if (!NUMERIC) {
+        const u32 n = dev::block_sum<u32, WG>(mine, red);
+        if (threadIdx.x == 0) nnz_row[r] = n;
+    }
+}
+
+}  // namespace spg
+}  // namespace clb

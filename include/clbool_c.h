@@ -1,0 +1,213 @@
+/*
+ * clbool_c.h -- C-ABI of the MI355X-native boolean sparse-matrix library.
+ *
+ * This is the drop-in boundary for clBool's SpGEMM path.  Every entry point
+ * names the reference interface it replaces (paths relative to the clBool
+ * source tree).  The C++ header clbool.hpp wraps these calls into clBool's own
+ * classes (clbool::Controls, matrix_dcsr, matrix_coo, matrix_csr, ...) so that
+ * code written against include/clbool.hpp of the reference compiles unchanged
+ * against this library.
+ *
+ * Conventions
+ *  - Every call returns clb_status: 0 on success, otherwise (Status + 1) where
+ *    Status is clBool's enum (src/core/error.hpp:130-151).  The message of the
+ *    last failure on the calling thread is returned by clb_last_error().
+ *  - Buffers are reference counted device allocations (they replace
+ *    cl::Buffer).  Matrices are plain structs of buffer handles plus sizes;
+ *    an output matrix is filled with NEW references that the caller owns and
+ *    eventually drops with clb_*_release.  Inputs are borrowed.  An output may
+ *    be the same struct as an input: the call reads all inputs first.
+ *  - All operations are ordered on the Controls' stream and complete before
+ *    the call returns, as in the reference (blocking reads and waits).
+ *  - Indices and sizes are uint32_t (src/core/matrix_base.hpp:12-16).
+ */
+#ifndef CLBOOL_C_H
+#define CLBOOL_C_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t clb_status;
+#define CLB_SUCCESS 0
+
+typedef struct clb_controls_s *clb_controls;
+typedef struct clb_buffer_s *clb_buffer;
+
+/* matrix_dcsr (src/core/matrix_dcsr.hpp:11-84) */
+typedef struct clb_dcsr {
+    clb_buffer rpt;   /* uint32[nzr + 1] */
+    clb_buffer rows;  /* uint32[nzr], strictly ascending */
+    clb_buffer cols;  /* uint32[nnz], ascending and unique within each row */
+    uint32_t nrows, ncols, nnz, nzr;
+} clb_dcsr;
+
+/* matrix_coo (src/core/matrix_coo.hpp:11-77) */
+typedef struct clb_coo {
+    clb_buffer rows, cols; /* uint32[nnz] each */
+    uint32_t nrows, ncols, nnz;
+} clb_coo;
+
+/* matrix_csr (src/core/matrix_csr.hpp:7-44) */
+typedef struct clb_csr {
+    clb_buffer rpt;   /* uint32[nrows + 1] */
+    clb_buffer cols;  /* uint32[nnz] */
+    uint32_t nrows, ncols, nnz;
+} clb_csr;
+
+/* Kernel-selection knobs of the SpGEMM (the hard-coded thresholds of
+ * src/dcsr/dcsr_matrix_multiplication_hash.cpp:13-47, made tunable). */
+typedef struct clb_tuning {
+    uint32_t hash_max_slots;       /* largest LDS hash table (slots, pow2)        */
+    uint32_t bitmap_min_products;  /* rows with at least this many products use
+                                      the LDS bitmap accumulator                  */
+    uint32_t bitmap_slot_ratio;    /* use a bitmap when its words <= ratio * the
+                                      hash slots the row would need               */
+    uint32_t tiny_max_products;    /* rows up to this many products use the
+                                      wave-level sort kernel                      */
+    uint32_t learned_margin_pct;   /* learned sizing: capacity = pred * (100 + m)%  */
+    uint32_t reserved[11];
+} clb_tuning;
+
+#define CLB_SPGEMM_PLAIN   0u  /* two-pass: symbolic count, scan, numeric      */
+#define CLB_SPGEMM_LEARNED 1u  /* one pass sized by the learned predictor,
+                                  overflowing rows re-run exactly              */
+
+typedef struct clb_spgemm_opts {
+    uint32_t mode;       /* CLB_SPGEMM_PLAIN or CLB_SPGEMM_LEARNED             */
+    uint32_t row_begin;  /* range of A's stored rows to multiply (row block);  */
+    uint32_t row_end;    /* row_end == 0 means all rows                       */
+    uint32_t timing;     /* nonzero: record per-stage times (HIP events)      */
+} clb_spgemm_opts;
+
+#define CLB_MAX_BINS 32
+typedef struct clb_spgemm_stats {
+    uint64_t products;      /* sum over rows of sum_{k in A_i} |B_k|          */
+    uint64_t nnz;           /* output nonzeros                               */
+    uint32_t nzr;           /* output nonempty rows                          */
+    uint32_t rows;          /* A rows processed                              */
+    uint32_t fallback_rows; /* learned mode: rows whose table overflowed     */
+    uint32_t n_bins;
+    uint32_t bin_rows[CLB_MAX_BINS];
+    uint64_t bin_products[CLB_MAX_BINS];
+    double   pred_abs_log2_err; /* learned mode: mean |log2(1+pred)-log2(1+nnz)| */
+    double   ms_setup, ms_symbolic, ms_numeric, ms_assemble, ms_total;
+} clb_spgemm_stats;
+
+typedef struct clb_device_info {
+    char     name[256];
+    char     arch[64];
+    uint32_t compute_units;
+    uint32_t max_wg_size;
+    uint32_t lds_bytes;
+    uint64_t global_mem_bytes;
+    uint32_t clock_khz;
+    uint32_t mem_clock_khz;
+    uint32_t mem_bus_width;
+} clb_device_info;
+
+/* ---- errors ------------------------------------------------------------ */
+/* Message of the last failure on this thread (Exception::what(),
+ * src/core/error.hpp:200-208). */
+const char *clb_last_error(void);
+
+/* ---- runtime (replaces src/core/controls.hpp, src/common/env.cpp) ------ */
+clb_status clb_device_count(uint32_t *count);
+/* create_controls (src/common/env.cpp:7-36): platform 0 is the HIP runtime;
+ * an unknown platform or device id fails with CLBOOL_INITIALIZATION_ERROR. */
+clb_status clb_controls_create(uint32_t platform_id, uint32_t device_id, clb_controls *out);
+clb_status clb_controls_destroy(clb_controls ctl);
+/* show_devices (src/common/env.cpp:38-56) as data */
+clb_status clb_device_info_get(uint32_t device_id, clb_device_info *info);
+clb_status clb_controls_sync(clb_controls ctl);
+clb_status clb_controls_max_wg_size(clb_controls ctl, uint32_t *out);
+clb_status clb_get_tuning(clb_controls ctl, clb_tuning *out);
+clb_status clb_set_tuning(clb_controls ctl, const clb_tuning *t);
+clb_status clb_last_stats(clb_controls ctl, clb_spgemm_stats *out);
+/* Drop the device allocation cache of this controls' device. */
+clb_status clb_pool_trim(clb_controls ctl);
+
+/* ---- buffers (replace cl::Buffer) --------------------------------------- */
+/* A zero-size buffer fails with CLBOOL_CREATE_BUFFER_ERROR, as cl::Buffer
+ * does (test/test_errors.cpp:38-42). */
+clb_status clb_buffer_create(clb_controls ctl, uint64_t bytes, clb_buffer *out);
+clb_status clb_buffer_from_host(clb_controls ctl, const void *src, uint64_t bytes, clb_buffer *out);
+clb_status clb_buffer_read(clb_controls ctl, clb_buffer buf, uint64_t offset, uint64_t bytes, void *dst);
+clb_status clb_buffer_write(clb_controls ctl, clb_buffer buf, uint64_t offset, uint64_t bytes, const void *src);
+clb_status clb_buffer_copy(clb_controls ctl, clb_buffer src, uint64_t src_offset, clb_buffer dst,
+                           uint64_t dst_offset, uint64_t bytes);
+clb_status clb_buffer_retain(clb_buffer buf);
+clb_status clb_buffer_release(clb_buffer buf);
+void *clb_buffer_device_ptr(clb_buffer buf);
+uint64_t clb_buffer_size(clb_buffer buf);
+
+/* ---- matrices ------------------------------------------------------------ */
+clb_status clb_dcsr_release(clb_dcsr *m);
+clb_status clb_coo_release(clb_coo *m);
+clb_status clb_csr_release(clb_csr *m);
+
+/* matrix_coo(controls, rows*, cols*, nrows, ncols, nnz, sorted, noDuplicates)
+ * (src/core/matrix_coo.cpp:17-47): upload, sort if !sorted, drop adjacent
+ * duplicates if !no_duplicates. */
+clb_status clb_coo_from_host(clb_controls ctl, const uint32_t *rows, const uint32_t *cols, uint32_t nrows,
+                             uint32_t ncols, uint32_t nnz, int sorted, int no_duplicates, clb_coo *out);
+/* matrix_coo(controls, nrows, ncols, nnz, rows&, cols&, sorted, noDuplicates)
+ * (src/core/matrix_coo.cpp:50-68). */
+clb_status clb_coo_from_buffers(clb_controls ctl, uint32_t nrows, uint32_t ncols, uint32_t nnz, clb_buffer rows,
+                                clb_buffer cols, int sorted, int no_duplicates, clb_coo *out);
+/* coo::sort_arrays (src/coo/coo_initialization.cpp:6-32): in-place
+ * lexicographic sort of (row, col) pairs. */
+clb_status clb_coo_sort_arrays(clb_controls ctl, clb_buffer rows, clb_buffer cols, uint32_t n);
+/* matrix_dcsr_from_cpu (src/common/matrices_conversions.cpp:93-108), with
+ * explicit nrows/ncols (the reference sets both to `size`). */
+clb_status clb_dcsr_from_host(clb_controls ctl, const uint32_t *rpt, const uint32_t *rows, const uint32_t *cols,
+                              uint32_t nzr, uint32_t nnz, uint32_t nrows, uint32_t ncols, clb_dcsr *out);
+/* csr_from_cpu (src/common/matrices_conversions.cpp:199-208) */
+clb_status clb_csr_from_host(clb_controls ctl, const uint32_t *rpt, const uint32_t *cols, uint32_t nrows,
+                             uint32_t ncols, uint32_t nnz, clb_csr *out);
+
+/* ---- conversions (src/common/matrices_conversions.hpp:9-26) -------------- */
+clb_status clb_coo_to_dcsr_shallow(clb_controls ctl, const clb_coo *a, clb_dcsr *out);  /* :78-91  */
+clb_status clb_dcsr_to_coo_shallow(clb_controls ctl, const clb_dcsr *a, clb_coo *out);  /* :43-57  */
+clb_status clb_dcsr_to_coo_deep(clb_controls ctl, const clb_dcsr *a, clb_coo *out);     /* :59-76  */
+clb_status clb_dcsr_to_csr(clb_controls ctl, const clb_dcsr *a, clb_csr *out);          /* :247-271 */
+clb_status clb_csr_to_dcsr(clb_controls ctl, const clb_csr *a, clb_dcsr *out);          /* :211-244 */
+
+/* ---- operations ----------------------------------------------------------- */
+/* dcsr::matrix_multiplication_hash (src/dcsr/dcsr_matrix_multiplication_hash.cpp:51-119):
+ * C = A * B over (OR, AND); throws CLBOOL_INVALID_ARGUMENT on a dimension
+ * mismatch; empty inputs give an empty (a.nrows x b.ncols) result. */
+clb_status clb_dcsr_spgemm_hash(clb_controls ctl, const clb_dcsr *a, const clb_dcsr *b, clb_dcsr *out);
+/* The same product with a mode (plain two-pass / learned one-pass), an
+ * optional row block of A, and statistics.  Output is bit-identical across
+ * modes. */
+clb_status clb_dcsr_spgemm(clb_controls ctl, const clb_dcsr *a, const clb_dcsr *b, const clb_spgemm_opts *opts,
+                           clb_dcsr *out, clb_spgemm_stats *stats);
+/* dcsr::kronecker_product (src/dcsr/dcsr_kronecker_product.cpp:6-65) */
+clb_status clb_dcsr_kronecker(clb_controls ctl, const clb_dcsr *a, const clb_dcsr *b, clb_dcsr *out);
+/* coo::matrix_addition (src/coo/coo_matrix_addition.cpp:32-73) */
+clb_status clb_coo_addition(clb_controls ctl, const clb_coo *a, const clb_coo *b, clb_coo *out);
+/* csr::matrix_addition (src/csr/csr_addition.cpp:26-168) */
+clb_status clb_csr_addition(clb_controls ctl, const clb_csr *a, const clb_csr *b, clb_csr *out);
+/* DCSR union C = A | B (the X += T step of a closure; row-wise merge). */
+clb_status clb_dcsr_addition(clb_controls ctl, const clb_dcsr *a, const clb_dcsr *b, clb_dcsr *out);
+
+/* ---- primitives (src/common/cl_operations.hpp:9-17) ---------------------- */
+/* prefix_sum (src/common/cl_operations.cpp:4-88): in-place exclusive scan of
+ * n uint32 values; *total receives the sum. */
+clb_status clb_prefix_sum(clb_controls ctl, clb_buffer buf, uint32_t n, uint32_t *total);
+/* fill_with (src/common/cl_operations.cpp:90-100) */
+clb_status clb_fill_with(clb_controls ctl, clb_buffer buf, uint32_t n, uint32_t value);
+
+/* KernelCache::get_kernel (src/core/kernel_cache.hpp:46-56): kernels are
+ * compiled ahead of time, so this only checks that `program`/`kernel` names
+ * one of them; unknown names fail with CLBOOL_NO_SUCH_PROGRAM or
+ * CLBOOL_CREATE_KERNEL_ERROR as in the reference. */
+clb_status clb_kernel_lookup(const char *program, const char *kernel);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CLBOOL_C_H */
