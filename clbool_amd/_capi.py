@@ -50,15 +50,23 @@ class SpgemmStats(C.Structure):
     _fields_ = [("products", C.c_uint64), ("nnz", C.c_uint64), ("nzr", C.c_uint32), ("rows", C.c_uint32),
                 ("fallback_rows", C.c_uint32), ("n_bins", C.c_uint32),
                 ("bin_rows", C.c_uint32 * CLB_MAX_BINS), ("bin_products", C.c_uint64 * CLB_MAX_BINS),
+                ("bin_a_nnz", C.c_uint64 * CLB_MAX_BINS), ("bin_out_nnz", C.c_uint64 * CLB_MAX_BINS),
                 ("pred_abs_log2_err", C.c_double),
                 ("ms_setup", C.c_double), ("ms_symbolic", C.c_double), ("ms_numeric", C.c_double),
-                ("ms_assemble", C.c_double), ("ms_total", C.c_double)]
+                ("ms_assemble", C.c_double), ("ms_total", C.c_double),
+                ("ms_kernel", C.c_double * 8), ("launches", C.c_uint32 * 8)]
+
+    _ARRAYS = ("bin_rows", "bin_products", "bin_a_nnz", "bin_out_nnz", "ms_kernel", "launches")
 
     def as_dict(self):
-        d = {f: getattr(self, f) for f, _ in self._fields_ if f not in ("bin_rows", "bin_products")}
-        d["bin_rows"] = list(self.bin_rows)
-        d["bin_products"] = list(self.bin_products)
+        d = {f: getattr(self, f) for f, _ in self._fields_ if f not in self._ARRAYS}
+        for f in self._ARRAYS:
+            d[f] = list(getattr(self, f))
         return d
+
+
+KERNEL_FAMILIES = ["sym_tiny", "sym_hash", "sym_bitmap", "num_tiny", "num_hash", "num_bitmap", "onepass",
+                   "fallback"]
 
 
 class DeviceInfo(C.Structure):

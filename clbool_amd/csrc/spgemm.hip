@@ -52,14 +52,14 @@ __global__ __launch_bounds__(256) void k_count(Mats g, u32 nrows, u32 *__restric
 
 // Histogram of bins.  key == nullptr: key = min(P, ncols) (symbolic sizing).
 // Rows in BIN_EMPTY get nnz_row = 0 when zero_empty is set.
-__global__ __launch_bounds__(256) void k_bin_hist(const u32 *__restrict__ P, const u32 *__restrict__ key, u32 n,
-                                                  BinCfg c, u32 *__restrict__ hist, u64 *__restrict__ binprod,
+__global__ __launch_bounds__(256) void k_bin_hist(Mats g, const u32 *__restrict__ P, const u32 *__restrict__ key,
+                                                  u32 n, BinCfg c, u32 *__restrict__ hist, u64 *__restrict__ sums,
                                                   u32 *__restrict__ nnz_row) {
     __shared__ u32 h[NBINS];
-    __shared__ unsigned long long hp[NBINS];
+    __shared__ unsigned long long hs[3][NBINS];  // products, A nonzeros, key (output nnz)
     if (threadIdx.x < NBINS) {
         h[threadIdx.x] = 0;
-        hp[threadIdx.x] = 0;
+        hs[0][threadIdx.x] = hs[1][threadIdx.x] = hs[2][threadIdx.x] = 0;
     }
     __syncthreads();
     const u32 i = blockIdx.x * 256 + threadIdx.x;
@@ -68,13 +68,16 @@ __global__ __launch_bounds__(256) void k_bin_hist(const u32 *__restrict__ P, con
         const u32 k = key ? key[i] : (p < c.ncols ? p : c.ncols);
         const u32 b = classify(p, k, c);
         atomicAdd(&h[b], 1u);
-        atomicAdd(&hp[b], (unsigned long long)p);
+        atomicAdd(&hs[0][b], (unsigned long long)p);
+        atomicAdd(&hs[1][b], (unsigned long long)(g.a_rpt[g.row0 + i + 1] - g.a_rpt[g.row0 + i]));
+        if (key) atomicAdd(&hs[2][b], (unsigned long long)k);
         if (b == BIN_EMPTY && nnz_row) nnz_row[i] = 0;
     }
     __syncthreads();
     if (threadIdx.x < NBINS && h[threadIdx.x]) {
         atomicAdd(&hist[threadIdx.x], h[threadIdx.x]);
-        atomicAdd(reinterpret_cast<unsigned long long *>(&binprod[threadIdx.x]), hp[threadIdx.x]);
+        for (int q = 0; q < 3; ++q)
+            atomicAdd(reinterpret_cast<unsigned long long *>(&sums[q * NBINS + threadIdx.x]), hs[q][threadIdx.x]);
     }
 }
 
@@ -184,45 +187,87 @@ void prepare(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &b, u32 row_
 }
 
 void bin_rows(clb_controls_s *ctl, const Setup &s, const u32 *key, u32 *nnz_row_zero, DevArray<u32> &perm,
-              DevArray<u32> &binbuf, u32 hist_host[NBINS + 1], u64 binprod_host[NBINS], bool readback_now) {
-    // binbuf: hist[NBINS] | cursor[NBINS+1] | (pad) | binprod as u64[NBINS]
-    binbuf.reset(ctl, 2 * NBINS + 2 + 2 * NBINS);
-    fill_u32(ctl, binbuf.ptr, binbuf.n, 0u);
-    u32 *hist = binbuf.ptr;
-    u32 *cursor = binbuf.ptr + NBINS;
-    u64 *binprod = reinterpret_cast<u64 *>(binbuf.ptr + 2 * NBINS + 2);
+              BinBuf &bb) {
+    bb.buf.reset(ctl, 2 * NBINS + 2 + 2 * 3 * NBINS);
+    fill_u32(ctl, bb.buf.ptr, bb.buf.n, 0u);
     perm.reset(ctl, s.nrows ? s.nrows : 1);
     const u32 grid = div_up(s.nrows, 256);
-    k_bin_hist<<<grid, 256, 0, ctl->stream>>>(s.prod.ptr, key, s.nrows, s.cfg, hist, binprod, nnz_row_zero);
+    k_bin_hist<<<grid, 256, 0, ctl->stream>>>(s.g, s.prod.ptr, key, s.nrows, s.cfg, bb.hist(), bb.sums(),
+                                              nnz_row_zero);
     CLB_LAUNCHED("bin_hist");
-    k_bin_offsets<<<1, 64, 0, ctl->stream>>>(hist, cursor);
+    k_bin_offsets<<<1, 64, 0, ctl->stream>>>(bb.hist(), bb.cursor());
     CLB_LAUNCHED("bin_offsets");
-    // scatter advances the cursor; keep the offsets: recompute from hist on host
-    k_bin_scatter<<<grid, 256, 0, ctl->stream>>>(s.prod.ptr, key, s.nrows, s.cfg, cursor, perm.ptr);
+    k_bin_scatter<<<grid, 256, 0, ctl->stream>>>(s.prod.ptr, key, s.nrows, s.cfg, bb.cursor(), perm.ptr);
     CLB_LAUNCHED("bin_scatter");
-    if (readback_now) {
-        u32 h[NBINS];
-        readback(ctl, h, hist, sizeof(h));
-        hist_host[0] = 0;
-        for (u32 b = 0; b < NBINS; ++b) hist_host[b + 1] = hist_host[b] + h[b];
-        if (binprod_host) readback(ctl, binprod_host, binprod, NBINS * sizeof(u64));
+}
+
+void read_bins(clb_controls_s *ctl, const BinBuf &bb, BinHost &h) {
+    u32 hist[NBINS];
+    readback(ctl, hist, bb.hist(), sizeof(hist));
+    h.off[0] = 0;
+    for (u32 b = 0; b < NBINS; ++b) h.off[b + 1] = h.off[b] + hist[b];
+    u64 sums[3 * NBINS];
+    readback(ctl, sums, bb.sums(), sizeof(sums));
+    for (u32 b = 0; b < NBINS; ++b) {
+        h.products[b] = sums[b];
+        h.a_nnz[b] = sums[NBINS + b];
+        h.out_nnz[b] = sums[2 * NBINS + b];
     }
+}
+
+void LaunchTimer::begin(hipStream_t st) {
+    if (!on) return;
+    CLB_HIP(hipEventCreate(&pending), EVENT_WAITING_ERROR);
+    CLB_HIP(hipEventRecord(pending, st), EVENT_WAITING_ERROR);
+}
+
+void LaunchTimer::end(int family, hipStream_t st) {
+    if (!on) return;
+    hipEvent_t e;
+    CLB_HIP(hipEventCreate(&e), EVENT_WAITING_ERROR);
+    CLB_HIP(hipEventRecord(e, st), EVENT_WAITING_ERROR);
+    recs.push_back({family, pending, e});
+    pending = nullptr;
+}
+
+void LaunchTimer::collect(clb_spgemm_stats &st) {
+    for (auto &r : recs) {
+        float m = 0;
+        if (hipEventElapsedTime(&m, r.a, r.b) == hipSuccess) {
+            st.ms_kernel[r.family] += m;
+            st.launches[r.family] += 1;
+        }
+    }
+}
+
+LaunchTimer::~LaunchTimer() {
+    for (auto &r : recs) {
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
+    if (pending) (void)hipEventDestroy(pending);
 }
 
 // Launch the per-bin kernels of one pass.  off[b]..off[b+1] is bin b's range
 // of perm.  numeric == false: write nnz_row; true: write columns.
 void launch_bins(clb_controls_s *ctl, const Setup &s, const u32 *perm, const u32 off[NBINS + 1], bool numeric,
-                 u32 *nnz_row, const u64 *out_off, u32 *out_cols, u32 *overflow) {
+                 u32 *nnz_row, const u64 *out_off, u32 *out_cols, u32 *overflow, LaunchTimer *lt) {
     fork_side(ctl);
     int k = 0;
     auto next = [&]() { return side_stream(ctl, k++); };
+    LaunchTimer dummy;
+    LaunchTimer &t = lt ? *lt : dummy;
+    const int fam = numeric ? CLB_KF_NUM_TINY : CLB_KF_SYM_TINY;
     // TINY
     {
         const u32 n = off[BIN_TINY + 1] - off[BIN_TINY];
         if (n) {
-            k_tiny<4><<<div_up(n, 4), 256, 0, next()>>>(s.g, perm + off[BIN_TINY], n, nnz_row, out_off,
-                                                          numeric ? out_cols : nullptr);
+            hipStream_t st = next();
+            t.begin(st);
+            k_tiny<4><<<div_up(n, 4), 256, 0, st>>>(s.g, perm + off[BIN_TINY], n, nnz_row, out_off,
+                                                      numeric ? out_cols : nullptr);
             CLB_LAUNCHED("tiny");
+            t.end(fam, st);
         }
     }
     // HASH
@@ -232,6 +277,7 @@ void launch_bins(clb_controls_s *ctl, const Setup &s, const u32 *perm, const u32
         if (!n) continue;
         const u32 *pp = perm + off[b];
         hipStream_t st = next();
+        t.begin(st);
 #define HK(BITS, WG)                                                                                          \
     case BITS:                                                                                                \
         if (numeric)                                                                                          \
@@ -244,6 +290,7 @@ void launch_bins(clb_controls_s *ctl, const Setup &s, const u32 *perm, const u32
         }
 #undef HK
         CLB_LAUNCHED("hash");
+        t.end(fam + 1, st);
     }
     // BITMAP (all classes in one launch, largest rows first)
     {
@@ -251,6 +298,7 @@ void launch_bins(clb_controls_s *ctl, const Setup &s, const u32 *perm, const u32
         if (n) {
             const u32 *pp = perm + off[BIN_BITMAP0];
             hipStream_t st = next();
+            t.begin(st);
 #define BK(WB, WG)                                                                                            \
     case WB:                                                                                                  \
         if (numeric)                                                                                          \
@@ -264,6 +312,7 @@ void launch_bins(clb_controls_s *ctl, const Setup &s, const u32 *perm, const u32
             }
 #undef BK
             CLB_LAUNCHED("bitmap");
+            t.end(fam + 2, st);
         }
     }
     join_side(ctl);
@@ -322,50 +371,44 @@ void spgemm_plain(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &b, u32
     const u32 nrows = row_end - row_begin;
     Setup s;
     prepare(ctl, a, b, row_begin, nrows, s);
-    DevArray<u32> perm, binbuf;
-    u32 off[NBINS + 1];
-    u64 bprod[NBINS];
+    LaunchTimer lt;
+    lt.on = timing;
+    DevArray<u32> perm;
+    BinBuf bb;
+    BinHost h1;
     DevArray<u32> nnz_row(ctl, nrows);
-    bin_rows(ctl, s, nullptr, nnz_row.ptr, perm, binbuf, off, bprod, false);
+    bin_rows(ctl, s, nullptr, nnz_row.ptr, perm, bb);
+    read_bins(ctl, bb, h1);
     u64 total_products = 0;
-    {
-        u32 h[NBINS];
-        readback(ctl, h, binbuf.ptr, sizeof(h));
-        off[0] = 0;
-        for (u32 k = 0; k < NBINS; ++k) off[k + 1] = off[k] + h[k];
-        readback(ctl, bprod, binbuf.ptr + 2 * NBINS + 2, sizeof(bprod));
-        readback(ctl, &total_products, s.scal.ptr, 8);
-    }
+    readback(ctl, &total_products, s.scal.ptr, 8);
     st.products = total_products;
     st.rows = nrows;
     st.n_bins = NBINS;
-    for (u32 k = 0; k < NBINS; ++k) {
-        st.bin_rows[k] = off[k + 1] - off[k];
-        st.bin_products[k] = bprod[k];
-    }
     tm.mark();  // 1
     if (total_products == 0) {
         out.nrows = a.nrows;
         out.ncols = b.ncols;
         return;
     }
-    launch_bins(ctl, s, perm.ptr, off, false, nnz_row.ptr, nullptr, nullptr,
-                reinterpret_cast<u32 *>(s.scal.ptr + 3));
+    launch_bins(ctl, s, perm.ptr, h1.off, false, nnz_row.ptr, nullptr, nullptr,
+                reinterpret_cast<u32 *>(s.scal.ptr + 3), &lt);
     tm.mark();  // 2
     DevArray<u64> offs;
     DevArray<u32> pos;
     scan_sizes(ctl, s, nnz_row.ptr, offs, pos);
-    DevArray<u32> perm2, binbuf2;
-    u32 off2[NBINS + 1];
-    bin_rows(ctl, s, nnz_row.ptr, nullptr, perm2, binbuf2, off2, nullptr, false);
-    u64 sc[4];
-    {
-        u32 h[NBINS];
-        readback(ctl, h, binbuf2.ptr, sizeof(h));
-        off2[0] = 0;
-        for (u32 k = 0; k < NBINS; ++k) off2[k + 1] = off2[k] + h[k];
-        readback(ctl, sc, s.scal.ptr, sizeof(sc));
+    DevArray<u32> perm2;
+    BinBuf bb2;
+    BinHost h2;
+    bin_rows(ctl, s, nnz_row.ptr, nullptr, perm2, bb2);
+    read_bins(ctl, bb2, h2);
+    for (u32 k = 0; k < NBINS; ++k) {
+        st.bin_rows[k] = h2.off[k + 1] - h2.off[k];
+        st.bin_products[k] = h2.products[k];
+        st.bin_a_nnz[k] = h2.a_nnz[k];
+        st.bin_out_nnz[k] = h2.out_nnz[k];
     }
+    u64 sc[4];
+    readback(ctl, sc, s.scal.ptr, sizeof(sc));
     tm.mark();  // 3
     if (sc[3]) raise(RUN_KERNEL_FAILURE, "hash table overflow in the symbolic pass");
     const u64 nnz = sc[1];
@@ -380,8 +423,8 @@ void spgemm_plain(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &b, u32
     if (nnz == 0) return;
     clb_buffer_s *cols = buffer_new(ctl, 4ull * nnz);
     try {
-        launch_bins(ctl, s, perm2.ptr, off2, true, nnz_row.ptr, offs.ptr, u32p(cols),
-                    reinterpret_cast<u32 *>(s.scal.ptr + 3));
+        launch_bins(ctl, s, perm2.ptr, h2.off, true, nnz_row.ptr, offs.ptr, u32p(cols),
+                    reinterpret_cast<u32 *>(s.scal.ptr + 3), &lt);
         tm.mark();  // 4
         assemble(ctl, s, a, nnz_row.ptr, offs.ptr, pos.ptr, nnz, nzr, cols, out);
         tm.mark();  // 5
@@ -396,6 +439,7 @@ void spgemm_plain(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &b, u32
     }
     st.nnz = nnz;
     st.nzr = nzr;
+    lt.collect(st);
     if (timing) {
         st.ms_setup = tm.ms(0, 1);
         st.ms_symbolic = tm.ms(1, 2);

@@ -18,12 +18,42 @@ struct Setup {
     BinCfg cfg{};
 };
 
+// HIP-event timing of individual kernel launches, grouped by family
+// (CLB_KF_* of clbool_c.h).  Disabled unless the caller asked for timing.
+struct LaunchTimer {
+    bool on = false;
+    struct Rec { int family; hipEvent_t a, b; };
+    std::vector<Rec> recs;
+    hipEvent_t pending = nullptr;
+    void begin(hipStream_t st);
+    void end(int family, hipStream_t st);
+    void collect(clb_spgemm_stats &st);  // after the streams are synchronised
+    ~LaunchTimer();
+};
+
+// Device layout of the binning scratch: hist[NBINS] | cursor[NBINS+1] | pad |
+// products[NBINS] | a_nnz[NBINS] | out_nnz[NBINS] (the last three as u64).
+struct BinBuf {
+    DevArray<u32> buf;
+    u32 *hist() const { return buf.ptr; }
+    u32 *cursor() const { return buf.ptr + NBINS; }
+    u64 *sums() const { return reinterpret_cast<u64 *>(buf.ptr + 2 * NBINS + 2); }
+};
+// Host copy of the binning: off[b]..off[b+1] is bin b's range of perm.
+struct BinHost {
+    u32 off[NBINS + 1];
+    u64 products[NBINS], a_nnz[NBINS], out_nnz[NBINS];
+};
+
 BinCfg make_bincfg(const clb_controls_s *ctl, u32 ncols);
 void prepare(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &b, u32 row_begin, u32 nrows, Setup &s);
+// Bins the rows of the block by (P, key) into perm (device only, no sync).
 void bin_rows(clb_controls_s *ctl, const Setup &s, const u32 *key, u32 *nnz_row_zero, DevArray<u32> &perm,
-              DevArray<u32> &binbuf, u32 hist_host[NBINS + 1], u64 binprod_host[NBINS], bool readback_now);
+              BinBuf &bb);
+// Reads the binning back (synchronises the stream).
+void read_bins(clb_controls_s *ctl, const BinBuf &bb, BinHost &h);
 void launch_bins(clb_controls_s *ctl, const Setup &s, const u32 *perm, const u32 off[NBINS + 1], bool numeric,
-                 u32 *nnz_row, const u64 *out_off, u32 *out_cols, u32 *overflow);
+                 u32 *nnz_row, const u64 *out_off, u32 *out_cols, u32 *overflow, LaunchTimer *lt);
 void scan_sizes(clb_controls_s *ctl, Setup &s, const u32 *nnz_row, DevArray<u64> &off, DevArray<u32> &pos);
 void assemble(clb_controls_s *ctl, const Setup &s, const clb_dcsr &a, const u32 *nnz_row, const u64 *off,
               const u32 *pos, u64 nnz, u32 nzr, clb_buffer_s *cols, clb_dcsr &out);

@@ -90,11 +90,27 @@ typedef struct clb_spgemm_stats {
     uint32_t rows;          /* A rows processed                              */
     uint32_t fallback_rows; /* learned mode: rows whose table overflowed     */
     uint32_t n_bins;
-    uint32_t bin_rows[CLB_MAX_BINS];
-    uint64_t bin_products[CLB_MAX_BINS];
+    uint32_t bin_rows[CLB_MAX_BINS];     /* rows per bin (numeric binning)          */
+    uint64_t bin_products[CLB_MAX_BINS]; /* intermediate products per bin           */
+    uint64_t bin_a_nnz[CLB_MAX_BINS];    /* A nonzeros of the bin's rows            */
+    uint64_t bin_out_nnz[CLB_MAX_BINS];  /* output nonzeros of the bin's rows       */
     double   pred_abs_log2_err; /* learned mode: mean |log2(1+pred)-log2(1+nnz)| */
     double   ms_setup, ms_symbolic, ms_numeric, ms_assemble, ms_total;
+    /* with opts.timing: HIP-event durations summed per kernel family, indexed
+     * by CLB_KF_*; launches[] counts the launches of each family */
+    double   ms_kernel[8];
+    uint32_t launches[8];
 } clb_spgemm_stats;
+
+/* kernel families of clb_spgemm_stats.ms_kernel */
+#define CLB_KF_SYM_TINY   0
+#define CLB_KF_SYM_HASH   1
+#define CLB_KF_SYM_BITMAP 2
+#define CLB_KF_NUM_TINY   3
+#define CLB_KF_NUM_HASH   4
+#define CLB_KF_NUM_BITMAP 5
+#define CLB_KF_ONEPASS    6  /* learned mode: the single sized pass          */
+#define CLB_KF_FALLBACK   7  /* learned mode: exact re-run of overflowed rows */
 
 typedef struct clb_device_info {
     char     name[256];
