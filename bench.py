@@ -97,7 +97,7 @@ def run_steps(cb, ctl, a_dev, blocks, mode, n, timing):
     agg = None
     t0 = time.perf_counter()
     for _ in range(n):
-        agg = {"products": 0, "nnz": 0, "ms_kernel": [0.0] * 8, "launches": [0] * 8,
+        agg = {"products": 0, "nnz": 0, "ms_kernel": [0.0] * 16, "launches": [0] * 16,
                "bin_rows": [0] * 32, "bin_products": [0] * 32, "bin_a_nnz": [0] * 32, "bin_out_nnz": [0] * 32,
                "fallback_rows": 0}
         for lo, hi in blocks:
@@ -110,6 +110,35 @@ def run_steps(cb, ctl, a_dev, blocks, mode, n, timing):
                 agg[k] = [x + y for x, y in zip(agg[k], st[k])]
     ctl.sync()
     return time.perf_counter() - t0, agg
+
+
+def train_predictor(cb, gen, ctl, log):
+    """Train the learned row-size predictor on the GPU from exact row sizes of
+    other generated instances (never the benchmark matrix itself)."""
+    t0 = time.perf_counter()
+    cb.predictor_reset(ctl, seed=1)
+    total = 0
+    recipe = [("rmat", 16, 16, 2), ("rmat", 17, 16, 3), ("rmat", 18, 16, 4), ("rmat", 17, 8, 5), ("rmat", 16, 32, 6),
+              ("uniform", 1 << 14, 20, 7), ("hyper", 22, 0, 8)]
+    for kind, x, y, seed in recipe:
+        if kind == "rmat":
+            r, c = gen.rmat(x, y, seed)
+            n = 1 << x
+        elif kind == "uniform":
+            r, c = gen.uniform_pairs(x * y, x, x, seed)
+            n = x
+        else:
+            r, c = gen.hypersparse_zipf(seed=seed, log_n=x, row_frac=0.01)
+            n = 1 << x
+        rpt, rows, cols = gen.to_dcsr(r, c)
+        m = cb.matrix_dcsr_from_cpu(ctl, rpt, rows, cols, n, n)
+        total = cb.predictor_harvest(ctl, m, m, max_rows=1 << 17)
+    st = cb.predictor_train(ctl, epochs=30, batch=4096, lr=3e-3, seed=1)
+    st["samples"] = total
+    st["wall_s"] = time.perf_counter() - t0
+    log(f"[bench] predictor trained on {total} rows: loss {st['loss_first']:.5f} -> {st['loss_last']:.5f} "
+        f"({st['ms']:.0f} ms of training)")
+    return st
 
 
 def cpu_baseline(oracle, A, target_s=12.0, seed=7):
@@ -158,7 +187,25 @@ def main():
     ap.add_argument("--block-products", type=float, default=float(1 << 31))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-validate", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--copy-probe", type=float, default=0,
+                    help="only run a device-to-device copy of this many bytes (4 times): the reference "
+                         "point that scales the FETCH_SIZE / WRITE_SIZE counters of a profiled run")
     args = ap.parse_args()
+    if args.copy_probe:
+        import clbool_amd as cb
+        ctl = cb.create_controls()
+        nbytes = int(args.copy_probe)
+        src = cb.Buffer.create(ctl, nbytes)
+        dst = cb.Buffer.create(ctl, nbytes)
+        cb.fill_with(ctl, src, nbytes // 4, 1)
+        for _ in range(4):
+            t0 = time.perf_counter()
+            cb.lib().clb_buffer_copy(ctl.handle, src.handle, 0, dst.handle, 0, nbytes)
+            dt = time.perf_counter() - t0
+            log(f"[copy-probe] {nbytes} bytes in {dt * 1e3:.3f} ms ({2 * nbytes / dt / 1e9:.1f} GB/s read+write)")
+        print(json.dumps({"copy_probe_bytes": nbytes}))
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU row-block sharding is a "
               "design note, not a measured path.", file=sys.stderr)
@@ -181,58 +228,58 @@ def main():
     total_products = int(prod.sum())
     log(f"[bench] A nnz={cols.size} nzr={rows.size} products={total_products} blocks={len(blocks)}")
 
-    modes = ["plain"]
-    if args.mode in ("auto", "learned"):
-        try:
-            cb.dcsr.spgemm(ctl, a_dev, a_dev, mode=cb.SPGEMM_LEARNED, row_begin=0, row_end=min(8, rows.size))
-            modes.append("learned")
-        except cb.ClboolError as e:
-            log(f"[bench] learned mode unavailable: {e}")
-            if args.mode == "learned":
-                raise
-    if args.mode == "plain":
-        modes = ["plain"]
+    modes = ["plain"] if args.mode == "plain" else (["learned"] if args.mode == "learned" else ["plain", "learned"])
+    code = {"plain": cb.SPGEMM_PLAIN, "learned": cb.SPGEMM_LEARNED}
+    train = train_predictor(cb, gen, ctl, log) if "learned" in modes else None
+
+    from clbool_amd import autotune as at
+    base_tuning = at.tuning_dict(ctl.tuning())
+    tuned = {}
+    for m in modes:
+        at.apply(ctl, base_tuning)
+        if not args.no_autotune:
+            sample = blocks[::max(1, len(blocks) // 3)][:4]
+            res = at.autotune(ctl, a_dev, a_dev, sample, mode=code[m], repeats=1, log=log)
+            tuned[m] = res["tuning"]
+            log(f"[bench] {m}: autotuned {res['tuning']} over {res['evaluated']} settings")
+        else:
+            tuned[m] = base_tuning
 
     import oracle  # checker only: validates sampled rows and times the CPU baseline
     A = oracle.Dcsr(rpt, rows, cols, n, n)
     validated = {}
-    if not args.no_validate:
-        for m in modes:
-            code = cb.SPGEMM_PLAIN if m == "plain" else cb.SPGEMM_LEARNED
-            validated[m] = validate_sample(cb, oracle, ctl, a_dev, A, blocks, code, np.random.default_rng(3))
-            log(f"[bench] {m}: {validated[m]} sampled rows bit-exact vs oracle")
-
     results = {}
     for m in modes:
-        code = cb.SPGEMM_PLAIN if m == "plain" else cb.SPGEMM_LEARNED
-        run_steps(cb, ctl, a_dev, blocks, code, args.warmup, False)
-        secs, agg = run_steps(cb, ctl, a_dev, blocks, code, args.steps, True)
+        at.apply(ctl, tuned[m])
+        if not args.no_validate:
+            validated[m] = validate_sample(cb, oracle, ctl, a_dev, A, blocks, code[m], np.random.default_rng(3))
+            log(f"[bench] {m}: {validated[m]} sampled rows bit-exact vs oracle")
+        run_steps(cb, ctl, a_dev, blocks, code[m], args.warmup, False)
+        secs, agg = run_steps(cb, ctl, a_dev, blocks, code[m], args.steps, True)
         ms = 1e3 * secs / args.steps
         results[m] = {"ms_per_step": ms, "products_per_s": agg["products"] / (ms / 1e3),
                       "nnz_per_s": agg["nnz"] / (ms / 1e3), "nnz": agg["nnz"], "agg": agg}
-        log(f"[bench] {m}: {ms:.2f} ms/step, {results[m]['products_per_s']:.3e} products/s")
+        log(f"[bench] {m}: {ms:.2f} ms/step, {results[m]['products_per_s']:.3e} products/s, "
+            f"fallback rows {agg['fallback_rows']}")
 
     best = "learned" if "learned" in results else "plain"
     R = results[best]
     agg = R["agg"]
-    # dominant kernel family over the timed region
+    # dominant kernel family over the timed region: pass x kind (see
+    # CLB_KF in include/clbool_c.h)
     fam_names = cb._capi.KERNEL_FAMILIES
     fam = int(np.argmax(agg["ms_kernel"]))
     launches = max(agg["launches"][fam], 1)
     avg_ms = agg["ms_kernel"][fam] / launches
     # algorithmic bytes of that family's launches: A rows (rpt 8 B/row,
     # a_cols 4 B + B row extent 8 B per A nonzero), 4 B per product read from
-    # B, 4 B per output column written
-    if fam in (2, 5):
-        bins = range(10, 32)
-    elif fam in (1, 4):
-        bins = range(2, 10)
-    else:
-        bins = range(1, 2)
-    numeric = fam >= 3
+    # B, 4 B per output column written (not in the symbolic pass)
+    pass_, kind = divmod(fam, 4)
+    bins = {0: range(1, 2), 1: range(2, 10), 2: range(10, 32)}.get(kind, range(0))
+    writes = pass_ != 0
     rows_f = sum(agg["bin_rows"][b] for b in bins)
     bytes_total = sum(12 * agg["bin_a_nnz"][b] + 4 * agg["bin_products"][b] +
-                      (4 * agg["bin_out_nnz"][b] if numeric else 0) for b in bins) + 8 * rows_f
+                      (4 * agg["bin_out_nnz"][b] if writes else 0) for b in bins) + 8 * rows_f
     bytes_per_launch = bytes_total / launches
     achieved = bytes_per_launch / (avg_ms / 1e3) / 1e9
     traffic = None
@@ -258,7 +305,8 @@ def main():
         "output_nnz_per_s": R["nnz_per_s"],
         "modes": {m: {"ms_per_step": v["ms_per_step"], "products_per_s": v["products_per_s"],
                       "ms_kernel": dict(zip(fam_names, v["agg"]["ms_kernel"])),
-                      "fallback_rows": v["agg"]["fallback_rows"]} for m, v in results.items()},
+                      "fallback_rows": v["agg"]["fallback_rows"], "tuning": tuned[m]} for m, v in results.items()},
+        "predictor_training": train,
         "validated_rows": validated,
         "roofline": roofline,
     }

@@ -396,3 +396,74 @@ dcsr = types.SimpleNamespace(matrix_multiplication_hash=_matrix_multiplication_h
                              kronecker_product=_kronecker_product, spgemm=_spgemm, addition=_dcsr_addition)
 coo = types.SimpleNamespace(matrix_addition=_coo_matrix_addition, sort_arrays=_coo_sort_arrays)
 csr = types.SimpleNamespace(matrix_addition=_csr_matrix_addition)
+
+
+# ---------------------------------------------------------------------------
+# Learned row-size predictor (no counterpart in the reference; see
+# include/clbool_c.h).  The kernels run on the GPU; these wrappers only move
+# arguments across the C-ABI.
+MLP_PARAMS = _capi.MLP_PARAMS
+
+
+def _f32(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float32))
+
+
+def predictor_reset(controls, seed=1):
+    _check(lib().clb_predictor_reset(controls.handle, seed))
+
+
+def predictor_harvest(controls, a: matrix_dcsr, b: matrix_dcsr, max_rows=1 << 20) -> int:
+    """Append (features, exact row size) samples of A * B; returns the total."""
+    n = C.c_uint64()
+    ac, bc = a._c(), b._c()
+    _check(lib().clb_predictor_harvest(controls.handle, C.byref(ac), C.byref(bc), max_rows, C.byref(n)))
+    return n.value
+
+
+def predictor_train(controls, epochs=20, batch=4096, lr=3e-3, seed=1) -> dict:
+    o = _capi.TrainOpts(epochs, batch, lr, seed)
+    s = _capi.TrainStats()
+    _check(lib().clb_predictor_train(controls.handle, C.byref(o), C.byref(s)))
+    return {f: getattr(s, f) for f, _ in s._fields_}
+
+
+def predictor_get_params(controls) -> np.ndarray:
+    p = np.zeros(MLP_PARAMS, np.float32)
+    _check(lib().clb_predictor_get_params(controls.handle, p.ctypes.data, MLP_PARAMS))
+    return p
+
+
+def predictor_set_params(controls, params):
+    p = _f32(params)
+    _check(lib().clb_predictor_set_params(controls.handle, p.ctypes.data, p.size))
+
+
+def predictor_predict(controls, a: matrix_dcsr, b: matrix_dcsr) -> np.ndarray:
+    out = np.zeros(a.nzr, np.float32)
+    ac, bc = a._c(), b._c()
+    _check(lib().clb_predictor_predict(controls.handle, C.byref(ac), C.byref(bc), out.ctypes.data))
+    return out
+
+
+def mlp_forward(controls, params, x) -> np.ndarray:
+    p, xx = _f32(params), _f32(x)
+    y = np.zeros(xx.shape[0], np.float32)
+    _check(lib().clb_mlp_forward(controls.handle, p.ctypes.data, xx.ctypes.data, xx.shape[0], y.ctypes.data))
+    return y
+
+
+def mlp_loss_grad(controls, params, x, t):
+    p, xx, tt = _f32(params), _f32(x), _f32(t)
+    g = np.zeros(MLP_PARAMS, np.float32)
+    loss = C.c_float()
+    _check(lib().clb_mlp_loss_grad(controls.handle, p.ctypes.data, xx.ctypes.data, tt.ctypes.data, xx.shape[0],
+                                   g.ctypes.data, C.byref(loss)))
+    return float(loss.value), g
+
+
+def mlp_adam(controls, params, m, v, g, lr, b1, b2, eps, step):
+    """One Adam step in place on float32 arrays params, m, v."""
+    gg = _f32(g)
+    _check(lib().clb_mlp_adam(controls.handle, params.ctypes.data, m.ctypes.data, v.ctypes.data, gg.ctypes.data,
+                              lr, b1, b2, eps, step))

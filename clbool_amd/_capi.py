@@ -54,7 +54,7 @@ class SpgemmStats(C.Structure):
                 ("pred_abs_log2_err", C.c_double),
                 ("ms_setup", C.c_double), ("ms_symbolic", C.c_double), ("ms_numeric", C.c_double),
                 ("ms_assemble", C.c_double), ("ms_total", C.c_double),
-                ("ms_kernel", C.c_double * 8), ("launches", C.c_uint32 * 8)]
+                ("ms_kernel", C.c_double * 16), ("launches", C.c_uint32 * 16), ("capacity", C.c_uint64)]
 
     _ARRAYS = ("bin_rows", "bin_products", "bin_a_nnz", "bin_out_nnz", "ms_kernel", "launches")
 
@@ -65,8 +65,21 @@ class SpgemmStats(C.Structure):
         return d
 
 
-KERNEL_FAMILIES = ["sym_tiny", "sym_hash", "sym_bitmap", "num_tiny", "num_hash", "num_bitmap", "onepass",
-                   "fallback"]
+KERNEL_PASSES = ["sym", "num", "onepass", "fallback"]
+KERNEL_KINDS = ["tiny", "hash", "bitmap", "emit"]
+KERNEL_FAMILIES = [f"{p}_{k}" for p in KERNEL_PASSES for k in KERNEL_KINDS]
+
+
+MLP_PARAMS = 1281
+
+
+class TrainOpts(C.Structure):
+    _fields_ = [("epochs", C.c_uint32), ("batch", C.c_uint32), ("lr", C.c_float), ("seed", C.c_uint32)]
+
+
+class TrainStats(C.Structure):
+    _fields_ = [("loss_first", C.c_double), ("loss_last", C.c_double), ("samples", C.c_uint64),
+                ("steps", C.c_uint32), ("ms", C.c_double)]
 
 
 class DeviceInfo(C.Structure):
@@ -120,6 +133,18 @@ SIGNATURES = [
     ("clb_coo_addition", C.c_int32, [ctl_t, C.POINTER(Coo), C.POINTER(Coo), C.POINTER(Coo)]),
     ("clb_csr_addition", C.c_int32, [ctl_t, C.POINTER(Csr), C.POINTER(Csr), C.POINTER(Csr)]),
     ("clb_dcsr_addition", C.c_int32, [ctl_t, C.POINTER(Dcsr), C.POINTER(Dcsr), C.POINTER(Dcsr)]),
+    ("clb_predictor_reset", C.c_int32, [ctl_t, C.c_uint32]),
+    ("clb_predictor_harvest", C.c_int32, [ctl_t, C.POINTER(Dcsr), C.POINTER(Dcsr), C.c_uint32,
+                                          C.POINTER(C.c_uint64)]),
+    ("clb_predictor_train", C.c_int32, [ctl_t, C.POINTER(TrainOpts), C.POINTER(TrainStats)]),
+    ("clb_predictor_get_params", C.c_int32, [ctl_t, C.c_void_p, C.c_uint32]),
+    ("clb_predictor_set_params", C.c_int32, [ctl_t, C.c_void_p, C.c_uint32]),
+    ("clb_predictor_predict", C.c_int32, [ctl_t, C.POINTER(Dcsr), C.POINTER(Dcsr), C.c_void_p]),
+    ("clb_mlp_forward", C.c_int32, [ctl_t, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
+    ("clb_mlp_loss_grad", C.c_int32, [ctl_t, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p,
+                                      C.c_void_p]),
+    ("clb_mlp_adam", C.c_int32, [ctl_t, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_float, C.c_float,
+                                 C.c_float, C.c_float, C.c_uint32]),
     ("clb_prefix_sum", C.c_int32, [ctl_t, buf_t, C.c_uint32, u32p]),
     ("clb_fill_with", C.c_int32, [ctl_t, buf_t, C.c_uint32, C.c_uint32]),
     ("clb_kernel_lookup", C.c_int32, [C.c_char_p, C.c_char_p]),

@@ -1,0 +1,123 @@
+"""Autotuner for the SpGEMM's bin thresholds, table sizes and kernel variants.
+
+Replaces the hard-coded thresholds of clBool's hash SpGEMM
+(src/dcsr/dcsr_matrix_multiplication_hash.cpp:13-47).  The tuner times the
+library on a sample of the caller's own workload (row blocks of A * B) and
+walks the knobs one at a time (coordinate descent), keeping a setting only if
+it is faster and gives the same result (nnz and a checksum of sampled rows).
+The winning knobs are stored per device (architecture and CU count) in a JSON
+file and can be reloaded without tuning again.
+"""
+from __future__ import annotations
+
+import json
+import os
+import time
+
+import numpy as np
+
+KNOBS = {
+    "reserved0": [0, 1],                       # bitmap emission variant
+    "bitmap_min_products": [4096, 8192, 16384, 32768, 65536],
+    "hash_max_slots": [4096, 8192, 16384],
+    "bitmap_slot_ratio": [1, 2, 4],
+    "tiny_max_products": [16, 32, 64],
+    "learned_margin_pct": [10, 25, 50],
+}
+
+
+def _get(t, k):
+    return t.reserved[0] if k == "reserved0" else getattr(t, k)
+
+
+def _set(t, k, v):
+    if k == "reserved0":
+        t.reserved[0] = v
+    else:
+        setattr(t, k, v)
+
+
+def tuning_dict(t) -> dict:
+    return {k: int(_get(t, k)) for k in KNOBS}
+
+
+def apply(controls, d: dict):
+    t = controls.tuning()
+    for k, v in d.items():
+        if k in KNOBS:
+            _set(t, k, int(v))
+    controls.set_tuning(t)
+
+
+def _checksum(cb, controls, c) -> tuple:
+    if c.nnz == 0:
+        return (0, 0)
+    k = min(c.nnz, 1 << 16)
+    head = c.cols_gpu.read(controls, k)
+    tail = c.cols_gpu.read(controls, k, offset=4 * (c.nnz - k))
+    return (int(c.nnz), int(head.astype(np.uint64).sum() ^ (tail.astype(np.uint64).sum() << np.uint64(1))))
+
+
+def autotune(controls, a, b, blocks, mode=0, repeats=2, knobs=None, log=None) -> dict:
+    """Coordinate descent over KNOBS on the given row blocks of A * B."""
+    import clbool_amd as cb
+
+    knobs = knobs or [k for k in KNOBS if mode == cb.SPGEMM_LEARNED or k != "learned_margin_pct"]
+
+    def run():
+        best = None
+        sums = []
+        for _ in range(repeats):
+            t0 = time.perf_counter()
+            sums = []
+            for lo, hi in blocks:
+                c, _ = cb.dcsr.spgemm(controls, a, b, mode=mode, row_begin=lo, row_end=hi)
+                sums.append(_checksum(cb, controls, c))
+                del c
+            dt = time.perf_counter() - t0
+            best = dt if best is None else min(best, dt)
+        return best, sums
+
+    cur = tuning_dict(controls.tuning())
+    run()  # warm the caches and the allocator
+    t_cur, ref = run()
+    history = [(dict(cur), t_cur)]
+    for k in knobs:
+        for v in KNOBS[k]:
+            if v == cur[k]:
+                continue
+            trial = dict(cur)
+            trial[k] = v
+            try:
+                apply(controls, trial)
+            except Exception:
+                continue
+            t, sums = run()
+            history.append((dict(trial), t))
+            if log:
+                log(f"[autotune] {k}={v}: {t * 1e3:.2f} ms (best {t_cur * 1e3:.2f})")
+            if sums != ref:
+                raise AssertionError(f"tuning {trial} changed the result")
+            if t < t_cur:
+                cur, t_cur = trial, t
+        apply(controls, cur)
+    return {"tuning": cur, "seconds": t_cur, "evaluated": len(history)}
+
+
+def cache_path(arch: str, cus: int, mode: str) -> str:
+    root = os.environ.get("CLBOOL_TUNING_DIR", os.path.join(os.path.expanduser("~"), ".cache", "clbool_amd"))
+    return os.path.join(root, f"tuning_{arch.split(':')[0]}_{cus}cu_{mode}.json")
+
+
+def save(path: str, result: dict):
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    with open(path, "w") as f:
+        json.dump(result, f, indent=1)
+
+
+def load(controls, path: str) -> bool:
+    if not os.path.exists(path):
+        return False
+    with open(path) as f:
+        apply(controls, json.load(f)["tuning"])
+    return True

@@ -44,10 +44,10 @@ __global__ __launch_bounds__(256) void k_merge_keys(const u64 *__restrict__ A, u
 
 // Keys of a CSR matrix, one wave per row.
 __global__ __launch_bounds__(256) void k_csr_keys(const u32 *__restrict__ rpt, const u32 *__restrict__ cols,
-                                                  u32 nrows, u64 *__restrict__ keys) {
+                                                  u32 nrows, u32 limit, u64 *__restrict__ keys) {
     const u32 r = blockIdx.x * 4 + dev::wave_id();
     if (r >= nrows) return;
-    const u32 e = rpt[r + 1];
+    const u32 e = rpt[r + 1] < limit ? rpt[r + 1] : limit;
     for (u32 j = rpt[r] + dev::lane(); j < e; j += 64) keys[j] = ((u64)r << 32) | cols[j];
 }
 
@@ -166,12 +166,19 @@ clb_status clb_csr_addition(clb_controls ctl, const clb_csr *a, const clb_csr *b
             *out = r;
             return;
         }
-        DevArray<u64> ka(ctl, a->nnz), kb(ctl, b->nnz);
-        k_csr_keys<<<div_up(a->nrows, 4), 256, 0, ctl->stream>>>(u32p(a->rpt), u32p(a->cols), a->nrows, ka.ptr);
-        k_csr_keys<<<div_up(b->nrows, 4), 256, 0, ctl->stream>>>(u32p(b->rpt), u32p(b->cols), b->nrows, kb.ptr);
+        // the rows [0, nrows) hold rpt[nrows] entries; like the reference's
+        // per-row kernels (csr_addition.cl), only those are merged
+        u32 na = 0, nb = 0;
+        readback(ctl, &na, u32p(a->rpt) + a->nrows, 4);
+        readback(ctl, &nb, u32p(b->rpt) + b->nrows, 4);
+        na = na < a->nnz ? na : a->nnz;
+        nb = nb < b->nnz ? nb : b->nnz;
+        DevArray<u64> ka(ctl, na ? na : 1), kb(ctl, nb ? nb : 1);
+        k_csr_keys<<<div_up(a->nrows, 4), 256, 0, ctl->stream>>>(u32p(a->rpt), u32p(a->cols), a->nrows, na, ka.ptr);
+        k_csr_keys<<<div_up(b->nrows, 4), 256, 0, ctl->stream>>>(u32p(b->rpt), u32p(b->cols), b->nrows, nb, kb.ptr);
         CLB_LAUNCHED("csr_keys");
         clb_buffer_s *rows = nullptr;
-        const u32 m = union_keys(ctl, ka.ptr, a->nnz, kb.ptr, b->nnz, &rows, &r.cols);
+        const u32 m = union_keys(ctl, ka.ptr, na, kb.ptr, nb, &rows, &r.cols);
         try {
             r.rpt = buffer_new(ctl, 4ull * (a->nrows + 1ull));
             k_rows_lower_bound<<<div_up(a->nrows + 1ull, 256), 256, 0, ctl->stream>>>(u32p(rows), m, a->nrows,

@@ -147,7 +147,8 @@ BinCfg make_bincfg(const clb_controls_s *ctl, u32 ncols) {
     return c;
 }
 
-void prepare(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &b, u32 row_begin, u32 nrows, Setup &s) {
+void prepare(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &b, u32 row_begin, u32 nrows, Setup &s,
+             bool count) {
     s.nrows = nrows;
     s.row0 = row_begin;
     s.mapn = b.nrows;
@@ -166,13 +167,13 @@ void prepare(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &b, u32 row_
     s.g.mapn = s.mapn;
     s.g.row0 = row_begin;
     s.g.ncols = b.ncols;
+    s.g.emit = ctl->tuning.reserved[0] ? EMIT_WORD_LANES : EMIT_LANE_BITS;
     s.cfg = make_bincfg(ctl, b.ncols);
     s.prod.reset(ctl, nrows);
     s.scal.reset(ctl, 8);  // [0] total products, [1] nnz total, [2] nzr total, [3] overflow flag
     fill_u64(ctl, s.scal.ptr, 8, 0ull);
-    // lanes per row from the mean row length of the block
-    const u64 nnz_blk = 0;  // unknown without a readback; use the matrix mean
-    (void)nnz_blk;
+    if (!count) return;
+    // lanes per row from the mean row length of the matrix
     const double mean = a.nzr ? (double)a.nnz / a.nzr : 1.0;
     int G = 1;
     while (G < 64 && G * 2 <= mean) G *= 2;
@@ -215,6 +216,38 @@ void read_bins(clb_controls_s *ctl, const BinBuf &bb, BinHost &h) {
     }
 }
 
+__global__ __launch_bounds__(256) void k_sum_by_bin(const u32 *__restrict__ perm, const u32 *__restrict__ off,
+                                                    u32 n, const u32 *__restrict__ nnz_row, u64 *__restrict__ sums) {
+    __shared__ unsigned long long hs[NBINS];
+    __shared__ u32 so[NBINS + 1];
+    if (threadIdx.x < NBINS) hs[threadIdx.x] = 0;
+    if (threadIdx.x <= NBINS) so[threadIdx.x] = off[threadIdx.x];
+    __syncthreads();
+    const u32 i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) {
+        u32 b = 0;
+        while (b + 1 < NBINS && so[b + 1] <= i) ++b;
+        const u32 v = nnz_row[perm[i]];
+        if (v != FAILED) atomicAdd(&hs[b], (unsigned long long)v);
+    }
+    __syncthreads();
+    if (threadIdx.x < NBINS && hs[threadIdx.x])
+        atomicAdd(reinterpret_cast<unsigned long long *>(&sums[threadIdx.x]), hs[threadIdx.x]);
+}
+
+void sum_by_bin(clb_controls_s *ctl, const u32 *perm, const BinHost &h, const u32 *nnz_row, u64 out[NBINS]) {
+    const u32 n = h.off[NBINS];
+    DevArray<u32> off(ctl, NBINS + 1);
+    DevArray<u64> sums(ctl, NBINS);
+    CLB_HIP(hipMemcpyAsync(off.ptr, h.off, 4 * (NBINS + 1), hipMemcpyHostToDevice, ctl->stream), WRITE_BUFFER_ERROR);
+    fill_u64(ctl, sums.ptr, NBINS, 0ull);
+    if (n) {
+        k_sum_by_bin<<<div_up(n, 256), 256, 0, ctl->stream>>>(perm, off.ptr, n, nnz_row, sums.ptr);
+        CLB_LAUNCHED("sum_by_bin");
+    }
+    readback(ctl, out, sums.ptr, NBINS * 8);
+}
+
 void LaunchTimer::begin(hipStream_t st) {
     if (!on) return;
     CLB_HIP(hipEventCreate(&pending), EVENT_WAITING_ERROR);
@@ -250,24 +283,34 @@ LaunchTimer::~LaunchTimer() {
 
 // Launch the per-bin kernels of one pass.  off[b]..off[b+1] is bin b's range
 // of perm.  numeric == false: write nnz_row; true: write columns.
-void launch_bins(clb_controls_s *ctl, const Setup &s, const u32 *perm, const u32 off[NBINS + 1], bool numeric,
-                 u32 *nnz_row, const u64 *out_off, u32 *out_cols, u32 *overflow, LaunchTimer *lt) {
+void launch_bins(clb_controls_s *ctl, const Setup &s, const u32 *perm, const u32 off[NBINS + 1], int mode,
+                 const Out &o, LaunchTimer *lt, int family) {
     fork_side(ctl);
     int k = 0;
     auto next = [&]() { return side_stream(ctl, k++); };
     LaunchTimer dummy;
     LaunchTimer &t = lt ? *lt : dummy;
-    const int fam = numeric ? CLB_KF_NUM_TINY : CLB_KF_SYM_TINY;
+    // family = the pass (CLB_KP_*); < 0 picks symbolic / numeric from the mode
+    const int pass = family >= 0 ? family : (mode == SYM ? (int)CLB_KP_SYMBOLIC : (int)CLB_KP_NUMERIC);
+    auto fam = [&](int kind) { return (int)CLB_KF(pass, kind); };
+#define DISPATCH3(KERNEL, GRID, WG, ST, ...)                                                       \
+    do {                                                                                          \
+        if (mode == SYM) KERNEL<__VA_ARGS__, SYM><<<GRID, WG, 0, ST>>>(s.g, pp, o);               \
+        else if (mode == NUM) KERNEL<__VA_ARGS__, NUM><<<GRID, WG, 0, ST>>>(s.g, pp, o);          \
+        else KERNEL<__VA_ARGS__, ONE><<<GRID, WG, 0, ST>>>(s.g, pp, o);                           \
+    } while (0)
     // TINY
     {
         const u32 n = off[BIN_TINY + 1] - off[BIN_TINY];
         if (n) {
             hipStream_t st = next();
+            const u32 *pp = perm + off[BIN_TINY];
             t.begin(st);
-            k_tiny<4><<<div_up(n, 4), 256, 0, st>>>(s.g, perm + off[BIN_TINY], n, nnz_row, out_off,
-                                                      numeric ? out_cols : nullptr);
+            if (mode == SYM) k_tiny<4, SYM><<<div_up(n, 4), 256, 0, st>>>(s.g, pp, n, o);
+            else if (mode == NUM) k_tiny<4, NUM><<<div_up(n, 4), 256, 0, st>>>(s.g, pp, n, o);
+            else k_tiny<4, ONE><<<div_up(n, 4), 256, 0, st>>>(s.g, pp, n, o);
             CLB_LAUNCHED("tiny");
-            t.end(fam, st);
+            t.end(fam(0), st);
         }
     }
     // HASH
@@ -278,19 +321,18 @@ void launch_bins(clb_controls_s *ctl, const Setup &s, const u32 *perm, const u32
         const u32 *pp = perm + off[b];
         hipStream_t st = next();
         t.begin(st);
-#define HK(BITS, WG)                                                                                          \
-    case BITS:                                                                                                \
-        if (numeric)                                                                                          \
-            k_hash<BITS, WG, true><<<n, WG, 0, st>>>(s.g, pp, nnz_row, out_off, out_cols, overflow);          \
-        else                                                                                                  \
-            k_hash<BITS, WG, false><<<n, WG, 0, st>>>(s.g, pp, nnz_row, out_off, out_cols, overflow);         \
-        break;
         switch (bits) {
-            HK(7, 64) HK(8, 64) HK(9, 64) HK(10, 128) HK(11, 128) HK(12, 256) HK(13, 512) HK(14, 1024)
+            case 7: DISPATCH3(k_hash, n, 64, st, 7, 64); break;
+            case 8: DISPATCH3(k_hash, n, 64, st, 8, 64); break;
+            case 9: DISPATCH3(k_hash, n, 64, st, 9, 64); break;
+            case 10: DISPATCH3(k_hash, n, 128, st, 10, 128); break;
+            case 11: DISPATCH3(k_hash, n, 128, st, 11, 128); break;
+            case 12: DISPATCH3(k_hash, n, 256, st, 12, 256); break;
+            case 13: DISPATCH3(k_hash, n, 512, st, 13, 512); break;
+            case 14: DISPATCH3(k_hash, n, 1024, st, 14, 1024); break;
         }
-#undef HK
         CLB_LAUNCHED("hash");
-        t.end(fam + 1, st);
+        t.end(fam(1), st);
     }
     // BITMAP (all classes in one launch, largest rows first)
     {
@@ -299,22 +341,23 @@ void launch_bins(clb_controls_s *ctl, const Setup &s, const u32 *perm, const u32
             const u32 *pp = perm + off[BIN_BITMAP0];
             hipStream_t st = next();
             t.begin(st);
-#define BK(WB, WG)                                                                                            \
-    case WB:                                                                                                  \
-        if (numeric)                                                                                          \
-            k_bitmap<WB, WG, true><<<n, WG, 0, st>>>(s.g, pp, nnz_row, out_off, out_cols);                    \
-        else                                                                                                  \
-            k_bitmap<WB, WG, false><<<n, WG, 0, st>>>(s.g, pp, nnz_row, out_off, out_cols);                   \
-        break;
             switch (s.cfg.bm_bits) {
-                BK(6, 64) BK(7, 64) BK(8, 64) BK(9, 256) BK(10, 256) BK(11, 256) BK(12, 512) BK(13, 512)
-                BK(14, 1024) BK(15, 1024)
+                case 6: DISPATCH3(k_bitmap, n, 64, st, 6, 64); break;
+                case 7: DISPATCH3(k_bitmap, n, 64, st, 7, 64); break;
+                case 8: DISPATCH3(k_bitmap, n, 64, st, 8, 64); break;
+                case 9: DISPATCH3(k_bitmap, n, 256, st, 9, 256); break;
+                case 10: DISPATCH3(k_bitmap, n, 256, st, 10, 256); break;
+                case 11: DISPATCH3(k_bitmap, n, 256, st, 11, 256); break;
+                case 12: DISPATCH3(k_bitmap, n, 512, st, 12, 512); break;
+                case 13: DISPATCH3(k_bitmap, n, 512, st, 13, 512); break;
+                case 14: DISPATCH3(k_bitmap, n, 1024, st, 14, 1024); break;
+                case 15: DISPATCH3(k_bitmap, n, 1024, st, 15, 1024); break;
             }
-#undef BK
             CLB_LAUNCHED("bitmap");
-            t.end(fam + 2, st);
+            t.end(fam(2), st);
         }
     }
+#undef DISPATCH3
     join_side(ctl);
 }
 
@@ -390,8 +433,8 @@ void spgemm_plain(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &b, u32
         out.ncols = b.ncols;
         return;
     }
-    launch_bins(ctl, s, perm.ptr, h1.off, false, nnz_row.ptr, nullptr, nullptr,
-                reinterpret_cast<u32 *>(s.scal.ptr + 3), &lt);
+    Out o1{nnz_row.ptr, nullptr, nullptr, nullptr, reinterpret_cast<u32 *>(s.scal.ptr + 3)};
+    launch_bins(ctl, s, perm.ptr, h1.off, SYM, o1, &lt, -1);
     tm.mark();  // 2
     DevArray<u64> offs;
     DevArray<u32> pos;
@@ -423,8 +466,8 @@ void spgemm_plain(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &b, u32
     if (nnz == 0) return;
     clb_buffer_s *cols = buffer_new(ctl, 4ull * nnz);
     try {
-        launch_bins(ctl, s, perm2.ptr, h2.off, true, nnz_row.ptr, offs.ptr, u32p(cols),
-                    reinterpret_cast<u32 *>(s.scal.ptr + 3), &lt);
+        Out o2{nnz_row.ptr, offs.ptr, u32p(cols), nullptr, reinterpret_cast<u32 *>(s.scal.ptr + 3)};
+        launch_bins(ctl, s, perm2.ptr, h2.off, NUM, o2, &lt, -1);
         tm.mark();  // 4
         assemble(ctl, s, a, nnz_row.ptr, offs.ptr, pos.ptr, nnz, nzr, cols, out);
         tm.mark();  // 5

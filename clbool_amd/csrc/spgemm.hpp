@@ -19,7 +19,7 @@ struct Setup {
 };
 
 // HIP-event timing of individual kernel launches, grouped by family
-// (CLB_KF_* of clbool_c.h).  Disabled unless the caller asked for timing.
+// (CLB_KF(pass, kind) of clbool_c.h).  Disabled unless the caller asked for timing.
 struct LaunchTimer {
     bool on = false;
     struct Rec { int family; hipEvent_t a, b; };
@@ -46,14 +46,21 @@ struct BinHost {
 };
 
 BinCfg make_bincfg(const clb_controls_s *ctl, u32 ncols);
-void prepare(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &b, u32 row_begin, u32 nrows, Setup &s);
+// B row map, kernel arguments and (count == true) per-row product counts.
+void prepare(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &b, u32 row_begin, u32 nrows, Setup &s,
+             bool count = true);
 // Bins the rows of the block by (P, key) into perm (device only, no sync).
 void bin_rows(clb_controls_s *ctl, const Setup &s, const u32 *key, u32 *nnz_row_zero, DevArray<u32> &perm,
               BinBuf &bb);
 // Reads the binning back (synchronises the stream).
 void read_bins(clb_controls_s *ctl, const BinBuf &bb, BinHost &h);
-void launch_bins(clb_controls_s *ctl, const Setup &s, const u32 *perm, const u32 off[NBINS + 1], bool numeric,
-                 u32 *nnz_row, const u64 *out_off, u32 *out_cols, u32 *overflow, LaunchTimer *lt);
+// Per-bin sums of nnz_row over the rows of each bin's range of perm (rows
+// marked FAILED count 0); the result is read back into out[NBINS].
+void sum_by_bin(clb_controls_s *ctl, const u32 *perm, const BinHost &h, const u32 *nnz_row, u64 out[NBINS]);
+// Launches the row kernels of every nonempty bin in `mode` (SYM, NUM or ONE)
+// on the side streams.  family < 0 times launches as sym_* / num_* by mode.
+void launch_bins(clb_controls_s *ctl, const Setup &s, const u32 *perm, const u32 off[NBINS + 1], int mode,
+                 const Out &o, LaunchTimer *lt, int family);
 void scan_sizes(clb_controls_s *ctl, Setup &s, const u32 *nnz_row, DevArray<u64> &off, DevArray<u32> &pos);
 void assemble(clb_controls_s *ctl, const Setup &s, const clb_dcsr &a, const u32 *nnz_row, const u64 *off,
               const u32 *pos, u64 nnz, u32 nzr, clb_buffer_s *cols, clb_dcsr &out);

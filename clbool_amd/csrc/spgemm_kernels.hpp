@@ -45,7 +45,14 @@ struct Mats {
     u32 mapn;             // entries of brange
     u32 row0;             // first A stored row of the block
     u32 ncols;            // columns of B (and C)
+    u32 emit;             // bitmap emission variant (EMIT_*)
 };
+
+// Bitmap emission variants (tuning knob, picked by the autotuner):
+//  EMIT_LANE_BITS: each lane walks the set bits of its own word.
+//  EMIT_WORD_LANES: 32 lanes share a word, one lane per bit; word values and
+//  word offsets are read from LDS so the unrolled iterations are independent.
+constexpr u32 EMIT_LANE_BITS = 0, EMIT_WORD_LANES = 1;
 
 struct BinCfg {
     u32 ncols;
@@ -76,11 +83,15 @@ __device__ __forceinline__ u32 classify(u32 P, u32 key, const BinCfg &c) {
 // ---------------------------------------------------------------------------
 // Product enumeration of one A row [js, je) by a workgroup of WG threads.
 // `big` is LDS scratch of 2*WG+1 words (unused when WG == 64).  f(col) is
-// called once per product, possibly from divergent lanes.
+// called once per product, possibly from divergent lanes.  Every path issues
+// UNROLL independent B-column loads per lane before applying f, so that a
+// wave keeps several gathers in flight (the kernels that use the largest LDS
+// tables run at one workgroup per CU and are otherwise latency bound).
+constexpr int UNROLL = 4;
+
 template <int WG, typename F>
 __device__ __forceinline__ void for_each_product(const Mats &g, u32 js, u32 je, u32 *big, F &&f) {
     const int l = dev::lane();
-    const int w = dev::wave_id();
     constexpr int NW = WG / dev::WAVE;
     for (u32 base = js; base < je; base += WG) {
         const u32 j = base + threadIdx.x;
@@ -110,38 +121,64 @@ __device__ __forceinline__ void for_each_product(const Mats &g, u32 js, u32 je, 
             lm &= lm - 1;
             const u32 sb = __shfl(bx, s, dev::WAVE);
             const u32 sl = __shfl(len, s, dev::WAVE);
-            for (u32 q = l; q < sl; q += dev::WAVE) f(g.b_cols[sb + q]);
+            for (u32 q = l; q < sl; q += UNROLL * dev::WAVE) {
+                u32 c[UNROLL];
+#pragma unroll
+                for (int u = 0; u < UNROLL; ++u) {
+                    const u32 qq = q + u * dev::WAVE;
+                    c[u] = qq < sl ? g.b_cols[sb + qq] : EMPTY;
+                }
+#pragma unroll
+                for (int u = 0; u < UNROLL; ++u)
+                    if (c[u] != EMPTY) f(c[u]);
+            }
         }
         // lane expansion of short rows
         const u32 sl = len < LONG_ROW ? len : 0;
         const u32 inc = dev::wave_incl_scan(sl);
         const u32 tot = __shfl(inc, dev::WAVE - 1, dev::WAVE);
         const u32 exc = inc - sl;
-        for (u32 p0 = 0; p0 < tot; p0 += dev::WAVE) {
-            const u32 p = p0 + l;
-            int s = 0;
-            u32 es = 0;
+        for (u32 p0 = 0; p0 < tot; p0 += UNROLL * dev::WAVE) {
+            u32 c[UNROLL];
 #pragma unroll
-            for (int step = 32; step > 0; step >>= 1) {
-                const u32 e = __shfl(exc, s + step, dev::WAVE);
-                if (e <= p) {
-                    s += step;
-                    es = e;
+            for (int u = 0; u < UNROLL; ++u) {
+                const u32 p = p0 + u * dev::WAVE + l;
+                int s = 0;
+                u32 es = 0;
+#pragma unroll
+                for (int step = 32; step > 0; step >>= 1) {
+                    const u32 e = __shfl(exc, s + step, dev::WAVE);
+                    if (e <= p) {
+                        s += step;
+                        es = e;
+                    }
                 }
+                const u32 sb = __shfl(bx, s, dev::WAVE);
+                c[u] = p < tot ? g.b_cols[sb + (p - es)] : EMPTY;
             }
-            const u32 sb = __shfl(bx, s, dev::WAVE);
-            if (p < tot) f(g.b_cols[sb + (p - es)]);
+#pragma unroll
+            for (int u = 0; u < UNROLL; ++u)
+                if (c[u] != EMPTY) f(c[u]);
         }
         if (NW > 1) {
             __syncthreads();
             const u32 nb = big[2 * WG];
             for (u32 e = 0; e < nb; ++e) {
                 const u32 sb = big[2 * e], sl2 = big[2 * e + 1];
-                for (u32 q = threadIdx.x; q < sl2; q += WG) f(g.b_cols[sb + q]);
+                for (u32 q = threadIdx.x; q < sl2; q += UNROLL * WG) {
+                    u32 c[UNROLL];
+#pragma unroll
+                    for (int u = 0; u < UNROLL; ++u) {
+                        const u32 qq = q + u * WG;
+                        c[u] = qq < sl2 ? g.b_cols[sb + qq] : EMPTY;
+                    }
+#pragma unroll
+                    for (int u = 0; u < UNROLL; ++u)
+                        if (c[u] != EMPTY) f(c[u]);
+                }
             }
             __syncthreads();
         }
-        (void)w;
     }
 }
 
@@ -165,13 +202,34 @@ __device__ __forceinline__ u32 hash_insert(u32 *table, u32 col) {
 }
 
 // ---------------------------------------------------------------------------
+// Pass modes shared by the row kernels.
+//  SYM: count the distinct columns of each row into nnz_row (symbolic pass).
+//  NUM: write each row's sorted columns at off[r]; nnz_row holds the exact
+//       count from the symbolic pass (numeric pass).
+//  ONE: single pass sized by a capacity cap[r]: write up to cap[r] sorted
+//       columns at off[r] and the count into nnz_row[r]; a row with more
+//       distinct columns than its capacity writes nothing, stores FAILED and
+//       bumps *overflow (its exact re-run happens in a second launch).
+constexpr int SYM = 0, NUM = 1, ONE = 2;
+constexpr u32 FAILED = 0xFFFFFFFFu;
+
+struct Out {
+    u32 *nnz_row;
+    const u64 *off;
+    u32 *cols;
+    const u32 *cap;
+    u32 *overflow;  // SYM/NUM: bit flags of inconsistencies; ONE: count of failed rows
+};
+
+template <int MODE>
+__device__ __forceinline__ u32 row_limit(const Out &o, u32 r) {
+    return MODE == NUM ? o.nnz_row[r] : (MODE == ONE ? o.cap[r] : 0u);
+}
+
 // TINY rows (P <= 64): one wave per row, one product per lane, register
 // bitonic sort across the wave, duplicates removed by a neighbour compare.
-// Symbolic (out_cols == nullptr) writes nnz_row; numeric writes the columns.
-template <int WAVES>
-__global__ __launch_bounds__(WAVES * 64) void k_tiny(Mats g, const u32 *__restrict__ perm, u32 count,
-                                                     u32 *__restrict__ nnz_row, const u64 *__restrict__ out_off,
-                                                     u32 *__restrict__ out_cols) {
+template <int WAVES, int MODE>
+__global__ __launch_bounds__(WAVES * 64) void k_tiny(Mats g, const u32 *__restrict__ perm, u32 count, Out o) {
     const u32 idx = blockIdx.x * WAVES + dev::wave_id();
     if (idx >= count) return;
     const int l = dev::lane();
@@ -214,30 +272,37 @@ __global__ __launch_bounds__(WAVES * 64) void k_tiny(Mats g, const u32 *__restri
     for (int k = 2; k <= 64; k <<= 1) {
 #pragma unroll
         for (int j = k >> 1; j > 0; j >>= 1) {
-            const u32 o = __shfl_xor(val, j, 64);
+            const u32 x = __shfl_xor(val, j, 64);
             const bool up = (l & k) == 0;
             const bool lower = (l & j) == 0;
-            const u32 mn = o < val ? o : val, mx = o < val ? val : o;
+            const u32 mn = x < val ? x : val, mx = x < val ? val : x;
             val = (lower == up) ? mn : mx;
         }
     }
     const u32 prev = __shfl_up(val, 1, 64);
     const bool keep = val != EMPTY && (l == 0 || prev != val);
     const unsigned long long km = __ballot(keep);
-    if (out_cols == nullptr) {
-        if (l == 0) nnz_row[r] = (u32)__popcll(km);
-    } else {
-        const u32 pos = (u32)__popcll(km & ((1ull << l) - 1ull));
-        if (keep && pos < nnz_row[r]) out_cols[out_off[r] + pos] = val;
+    const u32 n = (u32)__popcll(km);
+    if (MODE == SYM) {
+        if (l == 0) o.nnz_row[r] = n;
+        return;
     }
+    const u32 lim = row_limit<MODE>(o, r);
+    if (MODE == ONE) {
+        if (l == 0) o.nnz_row[r] = n <= lim ? n : FAILED;
+        if (n > lim) {
+            if (l == 0) atomicAdd(o.overflow, 1u);
+            return;
+        }
+    }
+    const u32 pos = (u32)__popcll(km & ((1ull << l) - 1ull));
+    if (keep && pos < lim) o.cols[o.off[r] + pos] = val;
 }
 
 // ---------------------------------------------------------------------------
 // HASH rows: LDS table of 2^BITS slots per workgroup.
-template <int BITS, int WG, bool NUMERIC>
-__global__ __launch_bounds__(WG) void k_hash(Mats g, const u32 *__restrict__ perm, u32 *__restrict__ nnz_row,
-                                             const u64 *__restrict__ out_off, u32 *__restrict__ out_cols,
-                                             u32 *__restrict__ overflow) {
+template <int BITS, int WG, int MODE>
+__global__ __launch_bounds__(WG) void k_hash(Mats g, const u32 *__restrict__ perm, Out o) {
     constexpr u32 T = 1u << BITS;
     __shared__ u32 table[T];
     __shared__ u32 big[2 * WG + 1];
@@ -254,11 +319,26 @@ __global__ __launch_bounds__(WG) void k_hash(Mats g, const u32 *__restrict__ per
         full |= rc == 2;
     });
     __syncthreads();
-    if (full) atomicOr(overflow, 1u);
-    if (!NUMERIC) {
+    if (MODE == SYM) {
+        if (full) atomicOr(o.overflow, 1u);
         const u32 n = dev::block_sum<u32, WG>(mine, red);
-        if (threadIdx.x == 0) nnz_row[r] = n;
+        if (threadIdx.x == 0) o.nnz_row[r] = n;
         return;
+    }
+    const u32 lim = row_limit<MODE>(o, r);
+    if (MODE == ONE) {
+        const u32 n = dev::block_sum<u32, WG>(mine, red);
+        const u32 anyfull = dev::block_sum<u32, WG>(full, red);
+        if (n > lim || anyfull) {
+            if (threadIdx.x == 0) {
+                o.nnz_row[r] = FAILED;
+                atomicAdd(o.overflow, 1u);
+            }
+            return;
+        }
+        if (threadIdx.x == 0) o.nnz_row[r] = n;
+    } else if (full) {
+        atomicOr(o.overflow, 1u);
     }
     // compact the live slots to the front of the table
     constexpr u32 PER = T / WG > 0 ? T / WG : 1;
@@ -280,21 +360,20 @@ __global__ __launch_bounds__(WG) void k_hash(Mats g, const u32 *__restrict__ per
     __syncthreads();
     dev::lds_bitonic_sort<WG>(table, n2);
     // never write past the row's slot, even if the passes disagreed
-    const u32 lim = n < nnz_row[r] ? n : nnz_row[r];
-    if (n != nnz_row[r] && threadIdx.x == 0) atomicOr(overflow, 2u);
-    u32 *out = out_cols + out_off[r];
-    for (u32 t = threadIdx.x; t < lim; t += WG) out[t] = table[t];
+    const u32 w = n < lim ? n : lim;
+    if (MODE == NUM && n != lim && threadIdx.x == 0) atomicOr(o.overflow, 2u);
+    u32 *out = o.cols + o.off[r];
+    for (u32 t = threadIdx.x; t < w; t += WG) out[t] = table[t];
 }
 
 // ---------------------------------------------------------------------------
 // BITMAP rows: an LDS bitmap of 2^WBITS words covers 32 * 2^WBITS columns;
 // wider column ranges are processed in slices.
-template <int WBITS, int WG, bool NUMERIC>
-__global__ __launch_bounds__(WG) void k_bitmap(Mats g, const u32 *__restrict__ perm, u32 *__restrict__ nnz_row,
-                                               const u64 *__restrict__ out_off, u32 *__restrict__ out_cols) {
+template <int WBITS, int WG, int MODE>
+__global__ __launch_bounds__(WG) void k_bitmap(Mats g, const u32 *__restrict__ perm, Out o) {
     constexpr u32 W = 1u << WBITS;
     constexpr u64 SPAN = 32ull * W;
-    __shared__ u32 bm[W];
+    __shared__ __align__(16) u32 bm[W];
     __shared__ u32 big[2 * WG + 1];
     __shared__ u32 red[WG / 64 + 1];
     const u32 r = perm[blockIdx.x];
@@ -303,42 +382,80 @@ __global__ __launch_bounds__(WG) void k_bitmap(Mats g, const u32 *__restrict__ p
     const u32 nslices = (u32)(((u64)g.ncols + SPAN - 1) / SPAN);
     u32 mine = 0;
     u64 written = 0;
-    u32 *out = NUMERIC ? out_cols + out_off[r] : nullptr;
-    const u64 lim = NUMERIC ? nnz_row[r] : 0;
+    u32 *out = MODE == SYM ? nullptr : o.cols + o.off[r];
+    const u64 lim = row_limit<MODE>(o, r);
     for (u32 s = 0; s < nslices; ++s) {
         const u64 base = (u64)s * SPAN;
-        for (u32 t = threadIdx.x; t < W; t += WG) bm[t] = 0;
+        {
+            uint4 *bm4 = reinterpret_cast<uint4 *>(bm);
+            for (u32 t = threadIdx.x; t < W / 4; t += WG) bm4[t] = make_uint4(0, 0, 0, 0);
+        }
         __syncthreads();
         for_each_product<WG>(g, js, je, big, [&](u32 col) {
             const u64 d = (u64)col - base;
             if (d < SPAN) atomicOr(&bm[(u32)(d >> 5)], 1u << (col & 31));
         });
         __syncthreads();
-        if (!NUMERIC) {
-            for (u32 t = threadIdx.x; t < W; t += WG) mine += __popc(bm[t]);
+        if (MODE == SYM) {
+            const uint4 *bm4 = reinterpret_cast<const uint4 *>(bm);
+            for (u32 t = threadIdx.x; t < W / 4; t += WG) {
+                const uint4 v = bm4[t];
+                mine += __popc(v.x) + __popc(v.y) + __popc(v.z) + __popc(v.w);
+            }
         } else {
-            // ordered emission, WG words per round
+            // Ordered emission, WG words per round; word offsets within the
+            // round come from a workgroup scan of popcounts.
+            const int l = dev::lane();
+            const u32 wbase = dev::wave_id() * dev::WAVE;
+            u32 *offs = big;  // free during emission: holds the round's word offsets
             for (u32 r0 = 0; r0 < W; r0 += WG) {
-                const u32 t = r0 + threadIdx.x;
-                u32 word = bm[t];
+                const u32 word = bm[r0 + threadIdx.x];
                 u32 tot;
                 const u32 p = dev::block_excl_scan<u32, WG>((u32)__popc(word), red, tot);
-                u64 q = written + p;
-                const u32 colbase = (u32)(base + 32ull * t);
-                while (word) {
-                    const u32 b = __ffs(word) - 1;
-                    if (q < lim) out[q] = colbase + b;
-                    ++q;
-                    word &= word - 1;
+                if (g.emit == EMIT_LANE_BITS) {
+                    u32 w = word;
+                    u64 q = written + p;
+                    const u32 colbase = (u32)(base + 32ull * (r0 + threadIdx.x));
+                    while (w) {
+                        const u32 b = __ffs(w) - 1;
+                        if (q < lim) out[q] = colbase + b;
+                        ++q;
+                        w &= w - 1;
+                    }
+                } else {
+                    offs[threadIdx.x] = p;
+                    __syncthreads();
+                    const u32 hi = (u32)l >> 5, bit = (u32)l & 31u;
+                    const u32 below = (1u << bit) - 1u;
+                    if (__ballot(word != 0)) {
+#pragma unroll 8
+                        for (u32 k = 0; k < 32; ++k) {
+                            const u32 wi = wbase + 2 * k + hi;
+                            const u32 wv = bm[r0 + wi];
+                            if ((wv >> bit) & 1u) {
+                                const u64 q = written + offs[wi] + (u32)__popc(wv & below);
+                                if (q < lim) out[q] = (u32)(base + 32ull * (r0 + wi)) + bit;
+                            }
+                        }
+                    }
+                    __syncthreads();
                 }
                 written += tot;
             }
         }
         __syncthreads();
     }
-    if (!NUMERIC) {
+    if (MODE == SYM) {
         const u32 n = dev::block_sum<u32, WG>(mine, red);
-        if (threadIdx.x == 0) nnz_row[r] = n;
+        if (threadIdx.x == 0) o.nnz_row[r] = n;
+    } else if (threadIdx.x == 0) {
+        if (MODE == ONE) {
+            // the capacity was too small: the partial output is discarded
+            o.nnz_row[r] = written <= lim ? (u32)written : FAILED;
+            if (written > lim) atomicAdd(o.overflow, 1u);
+        } else if (written != lim) {
+            atomicOr(o.overflow, 2u);
+        }
     }
 }
 

@@ -97,20 +97,22 @@ typedef struct clb_spgemm_stats {
     double   pred_abs_log2_err; /* learned mode: mean |log2(1+pred)-log2(1+nnz)| */
     double   ms_setup, ms_symbolic, ms_numeric, ms_assemble, ms_total;
     /* with opts.timing: HIP-event durations summed per kernel family, indexed
-     * by CLB_KF_*; launches[] counts the launches of each family */
-    double   ms_kernel[8];
-    uint32_t launches[8];
+     * by CLB_KF(pass, kind); launches[] counts the launches of each family */
+    double   ms_kernel[16];
+    uint32_t launches[16];
+    uint64_t capacity;      /* learned mode: columns reserved by the predicted sizes */
 } clb_spgemm_stats;
 
-/* kernel families of clb_spgemm_stats.ms_kernel */
-#define CLB_KF_SYM_TINY   0
-#define CLB_KF_SYM_HASH   1
-#define CLB_KF_SYM_BITMAP 2
-#define CLB_KF_NUM_TINY   3
-#define CLB_KF_NUM_HASH   4
-#define CLB_KF_NUM_BITMAP 5
-#define CLB_KF_ONEPASS    6  /* learned mode: the single sized pass          */
-#define CLB_KF_FALLBACK   7  /* learned mode: exact re-run of overflowed rows */
+/* kernel families of clb_spgemm_stats.ms_kernel: pass x kind */
+#define CLB_KP_SYMBOLIC 0u  /* plain mode, symbolic pass               */
+#define CLB_KP_NUMERIC  1u  /* plain mode, numeric pass                */
+#define CLB_KP_ONEPASS  2u  /* learned mode, the single sized pass     */
+#define CLB_KP_FALLBACK 3u  /* learned mode, exact re-run of overflows */
+#define CLB_KK_TINY     0u
+#define CLB_KK_HASH     1u
+#define CLB_KK_BITMAP   2u
+#define CLB_KK_EMIT     3u  /* bitmap emission / row compaction        */
+#define CLB_KF(pass, kind) ((pass) * 4u + (kind))
 
 typedef struct clb_device_info {
     char     name[256];
@@ -209,6 +211,47 @@ clb_status clb_coo_addition(clb_controls ctl, const clb_coo *a, const clb_coo *b
 clb_status clb_csr_addition(clb_controls ctl, const clb_csr *a, const clb_csr *b, clb_csr *out);
 /* DCSR union C = A | B (the X += T step of a closure; row-wise merge). */
 clb_status clb_dcsr_addition(clb_controls ctl, const clb_dcsr *a, const clb_dcsr *b, clb_dcsr *out);
+
+/* ---- learned row-size predictor (no counterpart in the reference) -------- */
+/* An MLP 5 -> 32 -> 32 -> 1 (fp32, ReLU) maps per-row features (log2 of
+ * 1 + products, A-row length, min and max B-row length, column span, each
+ * divided by 32) to log2(1 + nnz of the output row) / 32.  Parameters are
+ * CLB_MLP_PARAMS floats laid out W1[32][5], b1[32], W2[32][32], b2[32],
+ * w3[32], b3.  Training data are exact row sizes harvested from the plain
+ * symbolic pass; training runs Adam on the GPU with the library's own
+ * forward/backward kernels. */
+#define CLB_MLP_PARAMS 1281
+typedef struct clb_train_opts {
+    uint32_t epochs;
+    uint32_t batch;
+    float    lr;
+    uint32_t seed;
+} clb_train_opts;
+typedef struct clb_train_stats {
+    double   loss_first;  /* mean 0.5*(y-t)^2 of the first epoch */
+    double   loss_last;   /* ... of the last epoch               */
+    uint64_t samples;
+    uint32_t steps;
+    double   ms;
+} clb_train_stats;
+/* Initialise the parameters (seeded He initialisation) and clear the data. */
+clb_status clb_predictor_reset(clb_controls ctl, uint32_t seed);
+/* Append (features, exact row size) of up to max_rows rows of A * B. */
+clb_status clb_predictor_harvest(clb_controls ctl, const clb_dcsr *a, const clb_dcsr *b, uint32_t max_rows,
+                                 uint64_t *total_samples);
+clb_status clb_predictor_train(clb_controls ctl, const clb_train_opts *opts, clb_train_stats *stats);
+clb_status clb_predictor_get_params(clb_controls ctl, float *params, uint32_t n);
+clb_status clb_predictor_set_params(clb_controls ctl, const float *params, uint32_t n);
+/* Predicted output size of every stored row of A * B (a.nzr floats). */
+clb_status clb_predictor_predict(clb_controls ctl, const clb_dcsr *a, const clb_dcsr *b, float *pred_nnz);
+/* The MLP kernels on host data, for checking them against a CPU oracle:
+ * forward (y[n]), mean loss and its gradient (grad[CLB_MLP_PARAMS]), and one
+ * Adam step (in place on params, m, v). */
+clb_status clb_mlp_forward(clb_controls ctl, const float *params, const float *x, uint32_t n, float *y);
+clb_status clb_mlp_loss_grad(clb_controls ctl, const float *params, const float *x, const float *t, uint32_t n,
+                             float *grad, float *loss);
+clb_status clb_mlp_adam(clb_controls ctl, float *params, float *m, float *v, const float *grad, float lr, float b1,
+                        float b2, float eps, uint32_t step);
 
 /* ---- primitives (src/common/cl_operations.hpp:9-17) ---------------------- */
 /* prefix_sum (src/common/cl_operations.cpp:4-88): in-place exclusive scan of

@@ -1,0 +1,232 @@
+// test_dropin.cpp -- clBool's own test flows written against include/clbool.hpp
+// (the drop-in C++ API), checked against the C oracle.  Mirrors
+// test/src/operations.cpp (multiplication_hash, addition_coo, addition_csr,
+// kronecker_dcsr), test/src/procedures.cpp (pref_sum, bitonic_sort,
+// coo_constructor), test/src/conversions.cpp and test/test_errors.cpp.
+#include <clbool.hpp>
+
+#include <algorithm>
+#include <cstdio>
+#include <functional>
+#include <vector>
+
+#include "oracle.h"
+
+using namespace clbool;
+
+static int g_failed = 0, g_checks = 0;
+#define EXPECT(cond)                                                                 \
+    do {                                                                             \
+        ++g_checks;                                                                  \
+        if (!(cond)) {                                                               \
+            ++g_failed;                                                              \
+            std::fprintf(stderr, "%s:%d: expectation failed: %s\n", __FILE__, __LINE__, #cond); \
+        }                                                                            \
+    } while (0)
+
+static matrix_coo_cpu_pairs generate_coo_pairs_cpu(uint32_t pseudo_nnz, uint32_t max_size) {
+    orc_pairs *p = orc_generate_coo_pairs(pseudo_nnz, max_size);
+    matrix_coo_cpu_pairs out;
+    for (uint64_t i = 0; p && i < p->n; ++i) out.emplace_back(p->rows[i], p->cols[i]);
+    orc_free_pairs(p);
+    return out;
+}
+
+static matrix_dcsr_cpu coo_pairs_to_dcsr_cpu(const matrix_coo_cpu_pairs &m) {
+    if (m.empty()) return matrix_dcsr_cpu();
+    cpu_buffer r, c;
+    for (auto &x : m) {
+        r.push_back(x.first);
+        c.push_back(x.second);
+    }
+    orc_mat *d = orc_pairs_to_dcsr(r.data(), c.data(), r.size());
+    matrix_dcsr_cpu out(cpu_buffer(d->rpt, d->rpt + d->nzr + 1), cpu_buffer(d->rows, d->rows + d->nzr),
+                        cpu_buffer(d->cols, d->cols + d->nnz));
+    orc_free(d);
+    return out;
+}
+
+static matrix_dcsr_cpu matrix_multiplication_cpu(const matrix_dcsr_cpu &a, const matrix_dcsr_cpu &b) {
+    orc_mat *c = orc_spgemm_reference(a.rpt().data(), a.rows().data(), a.cols().data(), a.rows().size(),
+                                      b.rpt().data(), b.rows().data(), b.cols().data(), b.rows().size());
+    matrix_dcsr_cpu out(cpu_buffer(c->rpt, c->rpt + c->nzr + 1), cpu_buffer(c->rows, c->rows + c->nzr),
+                        cpu_buffer(c->cols, c->cols + c->nnz));
+    orc_free(c);
+    return out;
+}
+
+// compare_matrices (src/common/utils.cpp:42-56)
+static bool compare_matrices(Controls &controls, matrix_dcsr &m_gpu, const matrix_dcsr_cpu &m_cpu) {
+    if (m_gpu.nnz() != m_cpu.cols().size()) return false;
+    if (m_gpu.nnz() == 0) return true;
+    matrix_dcsr_cpu g = matrix_dcsr_from_gpu(controls, m_gpu);
+    return g.rpt() == m_cpu.rpt() && g.rows() == m_cpu.rows() && g.cols() == m_cpu.cols();
+}
+
+// test_multiplication_hash (test/src/operations.cpp:43-81)
+static bool test_multiplication_hash(Controls &controls, uint32_t size, uint32_t k) {
+    matrix_dcsr_cpu a_cpu = coo_pairs_to_dcsr_cpu(generate_coo_pairs_cpu(size * k, size));
+    matrix_dcsr_cpu c_cpu = matrix_multiplication_cpu(a_cpu, a_cpu);
+    matrix_dcsr a_gpu = matrix_dcsr_from_cpu(controls, a_cpu, size);
+    matrix_dcsr c_gpu;
+    dcsr::matrix_multiplication_hash(controls, c_gpu, a_gpu, a_gpu);
+    matrix_dcsr c_learned;
+    dcsr::matrix_multiplication_learned(controls, c_learned, a_gpu, a_gpu);
+    return compare_matrices(controls, c_gpu, c_cpu) && compare_matrices(controls, c_learned, c_cpu);
+}
+
+static bool test_addition_coo(Controls &controls, uint32_t size, uint32_t k_a, uint32_t k_b) {
+    matrix_coo_cpu_pairs a = generate_coo_pairs_cpu(size * k_a, size), b = generate_coo_pairs_cpu(size * k_b, size);
+    matrix_coo_cpu_pairs res;
+    std::merge(a.begin(), a.end(), b.begin(), b.end(), std::back_inserter(res));
+    res.erase(std::unique(res.begin(), res.end()), res.end());
+    cpu_buffer ar, ac, br, bc;
+    for (auto &x : a) ar.push_back(x.first), ac.push_back(x.second);
+    for (auto &x : b) br.push_back(x.first), bc.push_back(x.second);
+    matrix_coo ga(controls, ar.data(), ac.data(), size, size, ar.size());
+    matrix_coo gb(controls, br.data(), bc.data(), size, size, br.size());
+    matrix_coo out;
+    coo::matrix_addition(controls, out, ga, gb);
+    if (out.nnz() != res.size()) return false;
+    matrix_coo_cpu got = matrix_coo_from_gpu(controls, out);
+    for (size_t i = 0; i < res.size(); ++i)
+        if (got.rows()[i] != res[i].first || got.cols()[i] != res[i].second) return false;
+    return true;
+}
+
+static bool test_addition_csr(Controls &controls, uint32_t size, uint32_t k_a, uint32_t k_b) {
+    matrix_coo_cpu_pairs a = generate_coo_pairs_cpu(size * k_a, size), b = generate_coo_pairs_cpu(size * k_b, size);
+    matrix_coo_cpu_pairs res;
+    std::merge(a.begin(), a.end(), b.begin(), b.end(), std::back_inserter(res));
+    res.erase(std::unique(res.begin(), res.end()), res.end());
+    matrix_csr ga = csr_from_cpu(controls, csr_cpu_from_pairs(a, size, size));
+    matrix_csr gb = csr_from_cpu(controls, csr_cpu_from_pairs(b, size, size));
+    matrix_csr out;
+    csr::matrix_addition(controls, out, ga, gb);
+    matrix_csr_cpu expect = csr_cpu_from_pairs(res, size, size);
+    if (out.nnz() != expect.cols().size()) return false;
+    matrix_csr_cpu got = matrix_csr_from_gpu(controls, out);
+    return got.rpt() == expect.rpt() && got.cols() == expect.cols();
+}
+
+static bool test_kronecker_dcsr(Controls &controls, uint32_t size, uint32_t nnz) {
+    matrix_coo_cpu_pairs a = generate_coo_pairs_cpu(nnz, size);
+    matrix_coo_cpu_pairs res;
+    for (auto &x : a)
+        for (auto &y : a) res.emplace_back(x.first * size + y.first, x.second * size + y.second);
+    std::sort(res.begin(), res.end());
+    matrix_dcsr ga = matrix_dcsr_from_cpu(controls, coo_pairs_to_dcsr_cpu(a), size);
+    matrix_dcsr out;
+    dcsr::kronecker_product(controls, out, ga, ga);
+    return compare_matrices(controls, out, coo_pairs_to_dcsr_cpu(res));
+}
+
+static bool test_pref_sum(Controls &controls, uint32_t size) {
+    cpu_buffer vec(size + 1, 0);
+    orc_fastrandom r;
+    orc_fr_reset(&r, size);
+    for (uint32_t i = 0; i < size; ++i) vec[i] = orc_fr_next(&r, 0, 3);
+    clb_buffer h = nullptr;
+    CLBOOL_CALL(clb_buffer_from_host(controls.get(), vec.data(), 4ull * vec.size(), &h));
+    buffer vec_gpu(h);
+    uint32_t total = 0;
+    prefix_sum(controls, vec_gpu, total, size + 1);
+    uint32_t prev = 0;
+    cpu_buffer expect(vec.size());
+    for (size_t i = 0; i < vec.size(); ++i) expect[i] = prev, prev += vec[i];
+    cpu_buffer got(vec.size());
+    CLBOOL_CALL(clb_buffer_read(controls.get(), vec_gpu.get(), 0, 4ull * got.size(), got.data()));
+    return total == prev && got == expect;
+}
+
+int main() {
+    Controls controls = create_controls();
+    show_devices();
+    // the learned mode needs predictor parameters; a seeded initialisation is
+    // enough for exactness (rows whose capacity is too small are re-run)
+    CLBOOL_CALL(clb_predictor_reset(controls.get(), 1));
+
+    // README example (test/test_operations.cpp:279-319), now with assertions
+    {
+        std::vector<uint32_t> a_rows = {0, 0, 0, 2, 2, 4}, a_cols = {0, 1, 4, 2, 3, 2};
+        std::vector<uint32_t> b_rows = {1, 1, 2, 3, 3, 3, 5}, b_cols = {0, 4, 2, 2, 3, 4, 2};
+        matrix_coo a_coo(controls, a_rows.data(), a_cols.data(), 5, 5, 6);
+        matrix_coo b_coo(controls, b_rows.data(), b_cols.data(), 5, 5, 7);
+        matrix_coo c_coo;
+        coo::matrix_addition(controls, c_coo, a_coo, b_coo);
+        EXPECT(c_coo.nnz() == 12);
+        matrix_dcsr a_dcsr = coo_to_dcsr_shallow(controls, a_coo);
+        matrix_dcsr b_dcsr = coo_to_dcsr_shallow(controls, b_coo);
+        matrix_dcsr c_dcsr;
+        dcsr::matrix_multiplication_hash(controls, c_dcsr, a_dcsr, b_dcsr);
+        matrix_dcsr_cpu c = matrix_dcsr_from_gpu(controls, c_dcsr);
+        EXPECT((c.rpt() == cpu_buffer{0, 2, 5, 6}));
+        EXPECT((c.rows() == cpu_buffer{0, 2, 4}));
+        EXPECT((c.cols() == cpu_buffer{0, 4, 2, 3, 4, 2}));
+        // the output may alias an input
+        dcsr::matrix_multiplication_hash(controls, a_dcsr, a_dcsr, b_dcsr);
+        EXPECT(a_dcsr.nnz() == 6);
+        dcsr::kronecker_product(controls, c_dcsr, b_dcsr, b_dcsr);
+        EXPECT(c_dcsr.nnz() == 49 && c_dcsr.nrows() == 25);
+        // B stores a row with index 5 = nrows; its CSR form keeps 6 entries in
+        // rows 0..4, and the addition (like csr_addition.cl) merges those rows
+        matrix_csr a_csr = dcsr_to_csr(controls, b_dcsr);
+        matrix_csr c_csr;
+        csr::matrix_addition(controls, c_csr, a_csr, a_csr);
+        EXPECT(c_csr.nnz() == 6);
+        matrix_csr d_csr = dcsr_to_csr(controls, c_dcsr);
+        csr::matrix_addition(controls, c_csr, d_csr, a_csr);
+        EXPECT(c_csr.nnz() > 0);
+    }
+    for (uint32_t k = 50; k < 60; k += 5)
+        for (uint32_t size = 3000; size < 4000; size += 400) EXPECT(test_multiplication_hash(controls, size, k));
+    for (uint32_t size : {0u, 15u, 1000u, 4000u})
+        for (uint32_t ka : {0u, 3u, 30u})
+            for (uint32_t kb : {0u, 7u}) EXPECT(test_addition_coo(controls, size, ka, kb));
+    for (uint32_t size : {15u, 1000u, 7000u}) EXPECT(test_addition_csr(controls, size, 20, 35));
+    for (uint32_t k = 1; k < 4; ++k) EXPECT(test_kronecker_dcsr(controls, 10000, 1000 * k));
+    for (uint32_t size : {1u, 10000u, 590000u}) EXPECT(test_pref_sum(controls, size));
+    // coo_constructor (test/test_procedures.cpp:20-31)
+    {
+        std::vector<uint32_t> rows{0, 0, 1, 1, 3, 4, 4, 5, 5, 5, 6}, cols{1, 2, 0, 3, 5, 2, 2, 0, 3, 3, 5};
+        matrix_coo m(controls, rows.data(), cols.data(), 7, 7, rows.size(), true, false);
+        EXPECT(m.nnz() == rows.size() - 2);
+    }
+    // errors (test/test_errors.cpp)
+    {
+        bool threw = false;
+        try {
+            Controls c = create_controls(0, 64);
+        } catch (const Exception &e) {
+            threw = e.status() == CLBOOL_INITIALIZATION_ERROR;
+        }
+        EXPECT(threw);
+        threw = false;
+        try {
+            auto prepare_positions = kernel<buffer, buffer, uint32_t>("prepare_positions", "some name").set_work_size(25);
+            buffer x, y;
+            CLB_RUN(prepare_positions.run(controls, x, y, 25));
+        } catch (const Exception &e) {
+            threw = e.status() == CLBOOL_RUN_KERNEL_FAILURE;
+        }
+        EXPECT(threw);
+        threw = false;
+        try {
+            clb_buffer b = nullptr;
+            CLBOOL_CALL(clb_buffer_create(controls.get(), 0, &b));
+        } catch (const Exception &e) {
+            threw = e.status() == CLBOOL_CREATE_BUFFER_ERROR;
+        }
+        EXPECT(threw);
+        threw = false;
+        try {
+            matrix_dcsr a(4, 5), b(4, 5), c;
+            dcsr::matrix_multiplication_hash(controls, c, a, b);
+        } catch (const Exception &e) {
+            threw = e.status() == CLBOOL_INVALID_ARGUMENT;
+        }
+        EXPECT(threw);
+    }
+    std::printf("drop-in test: %d checks, %d failed\n", g_checks, g_failed);
+    return g_failed ? 1 : 0;
+}
