@@ -141,6 +141,43 @@ def train_predictor(cb, gen, ctl, log):
     return st
 
 
+def bench_closure(cb, gen, ctl, max_iters, log, seed=0):
+    """The 64-label closure on 2^18 vertices: 32 seeded triples, X |= Y * Z to
+    a fixed point or max_iters steps.  Per-triple plain SpGEMMs (the
+    reference's call pattern) against one batched learned SpGEMM per step;
+    both runs start from the same matrices and must end with identical ones."""
+    n = 1 << 18
+    labs = gen.closure_labels(n_labels=64, log_n=18, seed=seed)
+    triples = gen.closure_triples(64, 32, seed=seed + 1)
+    dcsr = [gen.to_dcsr(r, c) for r, c in labs]
+    out = {"labels": 64, "vertices": n, "triples": 32, "label_nnz": int(sum(d[2].size for d in dcsr))}
+    finals = {}
+    for name, mode, smode in (("per_triple_plain", cb.CLOSURE_PER_TRIPLE, cb.SPGEMM_PLAIN),
+                              ("batched_learned", cb.CLOSURE_BATCHED, cb.SPGEMM_LEARNED)):
+        mats = [cb.matrix_dcsr_from_cpu(ctl, rpt, rows, cols, n) for rpt, rows, cols in dcsr]
+        ctl.sync()
+        products, iters, steps_ms = 0, 0, []
+        t0 = time.perf_counter()
+        for it in range(max_iters):
+            ts = time.perf_counter()
+            st = cb.closure_step(ctl, mats, triples, mode=mode, spgemm_mode=smode)
+            steps_ms.append(1e3 * (time.perf_counter() - ts))
+            products += st["products"]
+            iters += 1
+            if st["changed"] == 0:
+                break
+        secs = time.perf_counter() - t0
+        finals[name] = [(m.nnz, int(m.cols_gpu.read(ctl, m.nnz).astype(np.uint64).sum()) if m.nnz else 0)
+                        for m in mats]
+        out[name] = {"iterations": iters, "seconds": secs, "products": products,
+                     "products_per_s": products / secs, "step_ms": [round(x, 2) for x in steps_ms],
+                     "fixed_point": st["changed"] == 0, "final_nnz": int(sum(m.nnz for m in mats))}
+        log(f"[bench] closure {name}: {iters} steps, {secs * 1e3:.1f} ms, {products / secs:.3e} products/s")
+    out["identical_results"] = finals["per_triple_plain"] == finals["batched_learned"]
+    out["speedup"] = out["per_triple_plain"]["seconds"] / out["batched_learned"]["seconds"]
+    return out
+
+
 def cpu_baseline(oracle, A, target_s=12.0, seed=7):
     """matrix_multiplication_cpu (coo_utils.cpp:470-514) restated in oracle/,
     single thread, on a bounded uniform sample of A's rows times the full B."""
@@ -188,6 +225,11 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-validate", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--closure-iters", type=int, default=6,
+                    help="steps of the 64-label closure workload reported beside the main metric (0: skip)")
+    ap.add_argument("--tuning-file", default=None,
+                    help="JSON written by a previous run (--save-tuning): use its knobs instead of tuning")
+    ap.add_argument("--save-tuning", default=None)
     ap.add_argument("--copy-probe", type=float, default=0,
                     help="only run a device-to-device copy of this many bytes (4 times): the reference "
                          "point that scales the FETCH_SIZE / WRITE_SIZE counters of a profiled run")
@@ -235,15 +277,23 @@ def main():
     from clbool_amd import autotune as at
     base_tuning = at.tuning_dict(ctl.tuning())
     tuned = {}
+    loaded = json.load(open(args.tuning_file)) if args.tuning_file else {}
     for m in modes:
         at.apply(ctl, base_tuning)
-        if not args.no_autotune:
+        if m in loaded:
+            tuned[m] = loaded[m]
+            log(f"[bench] {m}: tuning from {args.tuning_file}: {tuned[m]}")
+        elif not args.no_autotune:
             sample = blocks[::max(1, len(blocks) // 3)][:4]
             res = at.autotune(ctl, a_dev, a_dev, sample, mode=code[m], repeats=1, log=log)
             tuned[m] = res["tuning"]
             log(f"[bench] {m}: autotuned {res['tuning']} over {res['evaluated']} settings")
         else:
             tuned[m] = base_tuning
+
+    if args.save_tuning:
+        with open(args.save_tuning, "w") as f:
+            json.dump(tuned, f)
 
     import oracle  # checker only: validates sampled rows and times the CPU baseline
     A = oracle.Dcsr(rpt, rows, cols, n, n)
@@ -310,8 +360,11 @@ def main():
         "validated_rows": validated,
         "roofline": roofline,
     }
-    if "learned" in results:
+    if "learned" in results and "plain" in results:
         out["learned_vs_plain"] = results["learned"]["products_per_s"] / results["plain"]["products_per_s"]
+    if args.closure_iters > 0 and "learned" in modes:
+        at.apply(ctl, tuned.get("learned", base_tuning))
+        out["closure"] = bench_closure(cb, gen, ctl, args.closure_iters, log)
     if not args.no_cpu_baseline:
         log("[bench] timing the CPU baseline")
         out["cpu_baseline"] = cpu_baseline(oracle, A)

@@ -467,3 +467,27 @@ def mlp_adam(controls, params, m, v, g, lr, b1, b2, eps, step):
     gg = _f32(g)
     _check(lib().clb_mlp_adam(controls.handle, params.ctypes.data, m.ctypes.data, v.ctypes.data, gg.ctypes.data,
                               lr, b1, b2, eps, step))
+
+
+# ---------------------------------------------------------------------------
+# Closure step of CFPQ/RPQ evaluation (see include/clbool_c.h).
+CLOSURE_PER_TRIPLE = _capi.CLOSURE_PER_TRIPLE
+CLOSURE_BATCHED = _capi.CLOSURE_BATCHED
+
+
+def closure_step(controls, mats: list, triples, mode=CLOSURE_BATCHED, spgemm_mode=SPGEMM_PLAIN) -> dict:
+    """X |= Y * Z for every (X, Y, Z) index triple; replaces grown matrices in `mats`."""
+    arr = (_Dcsr * len(mats))()
+    for k, m in enumerate(mats):
+        arr[k] = m._c()
+        for b in (m.rpt_gpu, m.rows_gpu, m.cols_gpu):  # the struct owns one reference
+            if b is not None:
+                lib().clb_buffer_retain(b.handle)
+    flat = (C.c_uint32 * (3 * len(triples)))(*[int(v) for t in triples for v in t])
+    o = _capi.ClosureOpts(mode, spgemm_mode)
+    st = _capi.ClosureStats()
+    rc = lib().clb_closure_step(controls.handle, arr, len(mats), flat, len(triples), C.byref(o), C.byref(st))
+    for k in range(len(mats)):  # adopt what the struct holds now (old or new matrix)
+        mats[k] = matrix_dcsr._from_c(arr[k])
+    _check(rc)
+    return {f: getattr(st, f) for f, _ in st._fields_}

@@ -70,6 +70,17 @@ KERNEL_KINDS = ["tiny", "hash", "bitmap", "emit"]
 KERNEL_FAMILIES = [f"{p}_{k}" for p in KERNEL_PASSES for k in KERNEL_KINDS]
 
 
+class ClosureOpts(C.Structure):
+    _fields_ = [("mode", C.c_uint32), ("spgemm_mode", C.c_uint32), ("reserved", C.c_uint32 * 2)]
+
+
+class ClosureStats(C.Structure):
+    _fields_ = [("products", C.c_uint64), ("added", C.c_uint64), ("changed", C.c_uint32),
+                ("spgemm_calls", C.c_uint32), ("ms", C.c_double)]
+
+
+CLOSURE_PER_TRIPLE = 0
+CLOSURE_BATCHED = 1
 MLP_PARAMS = 1281
 
 
@@ -133,6 +144,8 @@ SIGNATURES = [
     ("clb_coo_addition", C.c_int32, [ctl_t, C.POINTER(Coo), C.POINTER(Coo), C.POINTER(Coo)]),
     ("clb_csr_addition", C.c_int32, [ctl_t, C.POINTER(Csr), C.POINTER(Csr), C.POINTER(Csr)]),
     ("clb_dcsr_addition", C.c_int32, [ctl_t, C.POINTER(Dcsr), C.POINTER(Dcsr), C.POINTER(Dcsr)]),
+    ("clb_closure_step", C.c_int32, [ctl_t, C.POINTER(Dcsr), C.c_uint32, C.POINTER(C.c_uint32), C.c_uint32,
+                                     C.POINTER(ClosureOpts), C.POINTER(ClosureStats)]),
     ("clb_predictor_reset", C.c_int32, [ctl_t, C.c_uint32]),
     ("clb_predictor_harvest", C.c_int32, [ctl_t, C.POINTER(Dcsr), C.POINTER(Dcsr), C.c_uint32,
                                           C.POINTER(C.c_uint64)]),

@@ -17,7 +17,8 @@ import time
 import numpy as np
 
 KNOBS = {
-    "reserved0": [0, 1],                       # bitmap emission variant
+    "reserved0": [0, 1],                       # in-LDS bitmap emission variant
+    "reserved1": [16, 64, 256, 1024, 65536],   # dense-bitmap dump threshold (65536: off)
     "bitmap_min_products": [4096, 8192, 16384, 32768, 65536],
     "hash_max_slots": [4096, 8192, 16384],
     "bitmap_slot_ratio": [1, 2, 4],
@@ -27,12 +28,12 @@ KNOBS = {
 
 
 def _get(t, k):
-    return t.reserved[0] if k == "reserved0" else getattr(t, k)
+    return t.reserved[int(k[8:])] if k.startswith("reserved") else getattr(t, k)
 
 
 def _set(t, k, v):
-    if k == "reserved0":
-        t.reserved[0] = v
+    if k.startswith("reserved"):
+        t.reserved[int(k[8:])] = v
     else:
         setattr(t, k, v)
 

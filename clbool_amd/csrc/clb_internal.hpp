@@ -27,6 +27,7 @@ namespace clb {
 
 using u32 = uint32_t;
 using u64 = uint64_t;
+using i64 = int64_t;
 
 // Status codes of clBool (src/core/error.hpp:130-151), same order.
 enum Status : int32_t {

@@ -300,15 +300,16 @@ __global__ void k_failed_keys(const u32 *__restrict__ nnz_row, const u32 *__rest
 __global__ __launch_bounds__(256) void k_compact_rows(const u32 *__restrict__ nnz_row, const u64 *__restrict__ off,
                                                       const u32 *__restrict__ temp, const u64 *__restrict__ toff,
                                                       const u32 *__restrict__ side, const u64 *__restrict__ soff,
-                                                      const u32 *__restrict__ key2, const float *__restrict__ pred,
-                                                      u32 *__restrict__ cols, double *__restrict__ err) {
+                                                      const u32 *__restrict__ key2, const u32 *__restrict__ isdense,
+                                                      const float *__restrict__ pred, u32 *__restrict__ cols,
+                                                      double *__restrict__ err) {
     const u32 r = blockIdx.x;
     const u32 n = nnz_row[r];
     if (threadIdx.x == 0 && pred && n) {
         const double e = fabs(log2(1.0 + (double)pred[r]) - log2(1.0 + (double)n));
         atomicAdd(err, e);
     }
-    if (!n) return;
+    if (!n || (isdense && isdense[r])) return;  // dense bitmap rows are emitted separately
     const u32 *src = (key2 && key2[r]) ? side + soff[r] : temp + toff[r];
     u32 *dst = cols + off[r];
     u32 t = threadIdx.x;
@@ -394,14 +395,16 @@ void spgemm_learned(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &b, u
     }
     DevArray<u32> temp(ctl, sc[4] ? sc[4] : 1);
     u32 *fails = reinterpret_cast<u32 *>(s.scal.ptr + 5);
-    Out o{nnz_row.ptr, toff.ptr, temp.ptr, cap.ptr, fails};
+    Out o{nnz_row.ptr, toff.ptr, temp.ptr, cap.ptr, fails, Dump{}};
+    DumpBufs dump;
+    setup_dump(ctl, s, h, dump, o);
     launch_bins(ctl, s, perm.ptr, h.off, ONE, o, &lt, CLB_KP_ONEPASS);
     u32 nfail = 0;
     readback(ctl, &nfail, fails, 4);
     st.fallback_rows = nfail;
     if (timing) {  // per-bin statistics of the one-pass launch (for its roofline)
         u64 outs[NBINS];
-        sum_by_bin(ctl, perm.ptr, h, nnz_row.ptr, outs);
+        sum_by_bin(ctl, perm.ptr, h, nrows, nnz_row.ptr, outs);
         for (u32 k = 0; k < NBINS; ++k) {
             st.bin_rows[k] = h.off[k + 1] - h.off[k];
             st.bin_products[k] = h.products[k];
@@ -430,7 +433,7 @@ void spgemm_learned(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &b, u
         readback(ctl, &side_total, s.scal.ptr + 6, 8);
         side.reset(ctl, side_total ? side_total : 1);
         u32 *fails2 = reinterpret_cast<u32 *>(s.scal.ptr + 7);
-        Out o2{nnz_row.ptr, soff.ptr, side.ptr, key2.ptr, fails2};
+        Out o2{nnz_row.ptr, soff.ptr, side.ptr, key2.ptr, fails2, Dump{}};
         launch_bins(ctl, sx, perm2.ptr, h2.off, ONE, o2, &lt, CLB_KP_FALLBACK);
         sx.cfg = keep;
         u32 nfail2 = 0;
@@ -453,9 +456,13 @@ void spgemm_learned(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &b, u
     if (nnz == 0) return;
     clb_buffer_s *cols = buffer_new(ctl, 4ull * nnz);
     try {
+        lt.begin(ctl->stream);
         k_compact_rows<<<nrows, 256, 0, ctl->stream>>>(nnz_row.ptr, offs.ptr, temp.ptr, toff.ptr, side.ptr, soff.ptr,
-                                                       nfail ? key2.ptr : nullptr, pred.ptr, u32p(cols), err.ptr);
+                                                       nfail ? key2.ptr : nullptr, dump.on ? dump.isdense.ptr : nullptr,
+                                                       pred.ptr, u32p(cols), err.ptr);
         CLB_LAUNCHED("compact_rows");
+        lt.end((int)CLB_KF(CLB_KP_FALLBACK, CLB_KK_EMIT), ctl->stream);
+        emit_dense(ctl, perm.ptr, h, dump, offs.ptr, u32p(cols), &lt, CLB_KP_ONEPASS);
         assemble(ctl, s, a, nnz_row.ptr, offs.ptr, pos.ptr, nnz, nzr, cols, out);
         double e = 0;
         readback(ctl, &e, err.ptr, 8);
@@ -561,7 +568,7 @@ clb_status clb_predictor_harvest(clb_controls ctl, const clb_dcsr *a, const clb_
             BinHost h;
             bin_rows(ctl, s, nullptr, nnz_row.ptr, perm, bb);
             read_bins(ctl, bb, h);
-            Out o{nnz_row.ptr, nullptr, nullptr, nullptr, reinterpret_cast<u32 *>(s.scal.ptr + 3)};
+            Out o{nnz_row.ptr, nullptr, nullptr, nullptr, reinterpret_cast<u32 *>(s.scal.ptr + 3), Dump{}};
             launch_bins(ctl, s, perm.ptr, h.off, SYM, o, nullptr, -1);
             const u32 stride = nrows > max_rows ? div_up(nrows, max_rows) : 1;
             const u32 take = div_up(nrows, stride);

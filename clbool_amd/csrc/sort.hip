@@ -152,6 +152,41 @@ __global__ void k_unique_scatter(const u64 *__restrict__ keys, u64 n, const u32 
     }
 }
 
+__global__ void k_unique_key_scatter(const u64 *__restrict__ keys, u64 n, const u32 *__restrict__ pos,
+                                     u64 *__restrict__ out) {
+    const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && pos[i] != pos[i + 1]) out[pos[i]] = keys[i];
+}
+
+u32 unique_keys(clb_controls_s *ctl, const u64 *keys, u64 n, DevArray<u64> &out) {
+    if (n == 0) {
+        out.reset(ctl, 1);
+        return 0;
+    }
+    DevArray<u32> pos(ctl, n + 1);
+    k_unique_flags<<<div_up(n + 1, 256), 256, 0, ctl->stream>>>(keys, n, pos.ptr);
+    CLB_LAUNCHED("unique_flags");
+    DevArray<u32> tot(ctl, 1);
+    exclusive_scan_u32(ctl, pos.ptr, pos.ptr, n + 1, tot.ptr);
+    u32 m = 0;
+    readback(ctl, &m, tot.ptr, 4);
+    out.reset(ctl, m ? m : 1);
+    k_unique_key_scatter<<<div_up(n, 256), 256, 0, ctl->stream>>>(keys, n, pos.ptr, out.ptr);
+    CLB_LAUNCHED("unique_key_scatter");
+    return m;
+}
+
+__global__ void k_rebase_keys(const u64 *__restrict__ src, u64 n, u64 delta_hi, u64 *__restrict__ dst) {
+    const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i] + delta_hi;  // wraps modulo 2^64 for negative deltas
+}
+
+void rebase_keys(clb_controls_s *ctl, const u64 *src, u64 n, i64 delta, u64 *dst) {
+    if (!n) return;
+    k_rebase_keys<<<div_up(n, 256), 256, 0, ctl->stream>>>(src, n, (u64)delta << 32, dst);
+    CLB_LAUNCHED("rebase_keys");
+}
+
 void pack_pairs(clb_controls_s *ctl, const u32 *rows, const u32 *cols, u64 n, u64 *keys) {
     if (!n) return;
     k_pack_pairs<<<div_up(n, 256), 256, 0, ctl->stream>>>(rows, cols, n, keys);

@@ -130,7 +130,94 @@ __global__ void k_set_positions(const u32 *__restrict__ nnz_row, const u64 *__re
     c_rows[q] = a_rows[row0 + i];
 }
 
+// Expands one 1024-word chunk of a dumped bitmap row into its columns at
+// their final positions: four rounds of 256 words, each staged in LDS so that
+// the global stores of a round are one contiguous run.
+// gpre = exclusive prefix of chunk_cnt over all slots.
+constexpr int EMIT_WG = 256;
+__global__ __launch_bounds__(EMIT_WG) void k_bitmap_emit(const uint4 *__restrict__ store, u32 nch,
+                                                         const u32 *__restrict__ chunk_cnt,
+                                                         const u64 *__restrict__ gpre, const u32 *__restrict__ list,
+                                                         const u32 *__restrict__ perm, const u64 *__restrict__ off,
+                                                         u32 words_per_row, u32 *__restrict__ cols) {
+    __shared__ u32 stage[EMIT_WG * 32];
+    __shared__ u32 red[EMIT_WG / 64 + 1];
+    const u32 li = blockIdx.x / nch, c = blockIdx.x % nch;
+    const u32 slot = list[li];
+    const u64 ci = (u64)slot * nch + c;
+    if (chunk_cnt[ci] == 0) return;
+    const u32 r = perm[slot];
+    u32 *out = cols + off[r] + (gpre[ci] - gpre[(u64)slot * nch]);
+    const u32 *words = reinterpret_cast<const u32 *>(store) + (u64)slot * words_per_row + (u64)c * EMIT_CHUNK;
+    const u32 colbase = c * EMIT_CHUNK * 32;
+    u32 w[EMIT_CHUNK / EMIT_WG];
+#pragma unroll
+    for (int k = 0; k < (int)(EMIT_CHUNK / EMIT_WG); ++k) w[k] = words[k * EMIT_WG + threadIdx.x];
+    u32 done = 0;
+#pragma unroll
+    for (int k = 0; k < (int)(EMIT_CHUNK / EMIT_WG); ++k) {
+        u32 tot;
+        u32 p = dev::block_excl_scan<u32, EMIT_WG>((u32)__popc(w[k]), red, tot);
+        u32 x = w[k];
+        const u32 cb = colbase + 32u * (k * EMIT_WG + threadIdx.x);
+        while (x) {
+            const u32 b = __ffs(x) - 1;
+            stage[p++] = cb + b;
+            x &= x - 1;
+        }
+        __syncthreads();
+        for (u32 t = threadIdx.x; t < tot; t += EMIT_WG) out[done + t] = stage[t];
+        done += tot;
+        __syncthreads();
+    }
+}
+
 // ---------------------------------------------------------------------------
+void setup_dump(clb_controls_s *ctl, const Setup &s, const BinHost &h, DumpBufs &d, Out &o) {
+    o.dump = Dump{};
+    d.on = false;
+    const u32 knob = ctl->tuning.reserved[1] ? ctl->tuning.reserved[1] : 256u;
+    if (s.cfg.bm_bits < 14 || knob >= 65536u) return;
+    const u64 nslices = ((u64)s.g.ncols + (32ull << s.cfg.bm_bits) - 1) / (32ull << s.cfg.bm_bits);
+    d.nbm = h.off[NBINS] - h.off[BIN_BITMAP0];
+    if (nslices != 1 || d.nbm == 0) return;
+    d.words = 1u << s.cfg.bm_bits;
+    d.nch = d.words / EMIT_CHUNK;
+    const u64 bytes = (u64)d.nbm * d.words * 4;
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
+        (void)hipGetLastError();
+        return;
+    }
+    if (bytes > free_b / 3) return;  // keep most of HBM for the operands and the output
+    d.store.reset(ctl, (u64)d.nbm * d.words / 4);
+    d.cnt.reset(ctl, (u64)d.nbm * d.nch);
+    d.list.reset(ctl, d.nbm);
+    d.count.reset(ctl, 1);
+    d.isdense.reset(ctl, s.nrows);
+    fill_u32(ctl, d.cnt.ptr, d.cnt.n, 0);
+    fill_u32(ctl, d.count.ptr, 1, 0);
+    fill_u32(ctl, d.isdense.ptr, s.nrows, 0);
+    o.dump = Dump{d.store.ptr, d.cnt.ptr, d.list.ptr, d.count.ptr, d.isdense.ptr, (d.words / 1024u) * knob};
+    d.on = true;
+}
+
+void emit_dense(clb_controls_s *ctl, const u32 *perm, const BinHost &h, DumpBufs &d, const u64 *off, u32 *cols,
+                LaunchTimer *lt, int pass) {
+    if (!d.on) return;
+    u32 count = 0;
+    readback(ctl, &count, d.count.ptr, 4);
+    if (!count) return;
+    DevArray<u64> gpre(ctl, d.cnt.n + 1);
+    exclusive_scan_u32_to_u64(ctl, d.cnt.ptr, gpre.ptr, d.cnt.n, nullptr);
+    if (lt) lt->begin(ctl->stream);
+    k_bitmap_emit<<<(u64)count * d.nch, EMIT_WG, 0, ctl->stream>>>(d.store.ptr, d.nch, d.cnt.ptr, gpre.ptr,
+                                                                    d.list.ptr, perm + h.off[BIN_BITMAP0], off,
+                                                                    d.words, cols);
+    CLB_LAUNCHED("bitmap_emit");
+    if (lt) lt->end((int)CLB_KF(pass, CLB_KK_EMIT), ctl->stream);
+}
+
 BinCfg make_bincfg(const clb_controls_s *ctl, u32 ncols) {
     BinCfg c{};
     c.ncols = ncols;
@@ -216,18 +303,22 @@ void read_bins(clb_controls_s *ctl, const BinBuf &bb, BinHost &h) {
     }
 }
 
+// Sums nnz_row over the rows of bins 1..NBINS-1 (perm[off[1] .. off[NBINS]));
+// the EMPTY bin's range of perm is never written by the scatter.
 __global__ __launch_bounds__(256) void k_sum_by_bin(const u32 *__restrict__ perm, const u32 *__restrict__ off,
-                                                    u32 n, const u32 *__restrict__ nnz_row, u64 *__restrict__ sums) {
+                                                    u32 nrows, const u32 *__restrict__ nnz_row,
+                                                    u64 *__restrict__ sums) {
     __shared__ unsigned long long hs[NBINS];
     __shared__ u32 so[NBINS + 1];
     if (threadIdx.x < NBINS) hs[threadIdx.x] = 0;
     if (threadIdx.x <= NBINS) so[threadIdx.x] = off[threadIdx.x];
     __syncthreads();
-    const u32 i = blockIdx.x * 256 + threadIdx.x;
-    if (i < n) {
-        u32 b = 0;
+    const u32 i = so[1] + blockIdx.x * 256 + threadIdx.x;
+    if (i < so[NBINS]) {
+        u32 b = 1;
         while (b + 1 < NBINS && so[b + 1] <= i) ++b;
-        const u32 v = nnz_row[perm[i]];
+        const u32 row = perm[i];
+        const u32 v = row < nrows ? nnz_row[row] : 0u;
         if (v != FAILED) atomicAdd(&hs[b], (unsigned long long)v);
     }
     __syncthreads();
@@ -235,14 +326,15 @@ __global__ __launch_bounds__(256) void k_sum_by_bin(const u32 *__restrict__ perm
         atomicAdd(reinterpret_cast<unsigned long long *>(&sums[threadIdx.x]), hs[threadIdx.x]);
 }
 
-void sum_by_bin(clb_controls_s *ctl, const u32 *perm, const BinHost &h, const u32 *nnz_row, u64 out[NBINS]) {
-    const u32 n = h.off[NBINS];
+void sum_by_bin(clb_controls_s *ctl, const u32 *perm, const BinHost &h, u32 nrows, const u32 *nnz_row,
+                u64 out[NBINS]) {
+    const u32 n = h.off[NBINS] - h.off[1];
     DevArray<u32> off(ctl, NBINS + 1);
     DevArray<u64> sums(ctl, NBINS);
     CLB_HIP(hipMemcpyAsync(off.ptr, h.off, 4 * (NBINS + 1), hipMemcpyHostToDevice, ctl->stream), WRITE_BUFFER_ERROR);
     fill_u64(ctl, sums.ptr, NBINS, 0ull);
     if (n) {
-        k_sum_by_bin<<<div_up(n, 256), 256, 0, ctl->stream>>>(perm, off.ptr, n, nnz_row, sums.ptr);
+        k_sum_by_bin<<<div_up(n, 256), 256, 0, ctl->stream>>>(perm, off.ptr, nrows, nnz_row, sums.ptr);
         CLB_LAUNCHED("sum_by_bin");
     }
     readback(ctl, out, sums.ptr, NBINS * 8);
@@ -433,7 +525,7 @@ void spgemm_plain(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &b, u32
         out.ncols = b.ncols;
         return;
     }
-    Out o1{nnz_row.ptr, nullptr, nullptr, nullptr, reinterpret_cast<u32 *>(s.scal.ptr + 3)};
+    Out o1{nnz_row.ptr, nullptr, nullptr, nullptr, reinterpret_cast<u32 *>(s.scal.ptr + 3), Dump{}};
     launch_bins(ctl, s, perm.ptr, h1.off, SYM, o1, &lt, -1);
     tm.mark();  // 2
     DevArray<u64> offs;
@@ -466,8 +558,11 @@ void spgemm_plain(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &b, u32
     if (nnz == 0) return;
     clb_buffer_s *cols = buffer_new(ctl, 4ull * nnz);
     try {
-        Out o2{nnz_row.ptr, offs.ptr, u32p(cols), nullptr, reinterpret_cast<u32 *>(s.scal.ptr + 3)};
+        Out o2{nnz_row.ptr, offs.ptr, u32p(cols), nullptr, reinterpret_cast<u32 *>(s.scal.ptr + 3), Dump{}};
+        DumpBufs dump;
+        setup_dump(ctl, s, h2, dump, o2);
         launch_bins(ctl, s, perm2.ptr, h2.off, NUM, o2, &lt, -1);
+        emit_dense(ctl, perm2.ptr, h2, dump, offs.ptr, u32p(cols), &lt, CLB_KP_NUMERIC);
         tm.mark();  // 4
         assemble(ctl, s, a, nnz_row.ptr, offs.ptr, pos.ptr, nnz, nzr, cols, out);
         tm.mark();  // 5

@@ -45,6 +45,21 @@ struct BinHost {
     u64 products[NBINS], a_nnz[NBINS], out_nnz[NBINS];
 };
 
+// Scratch of the dense-bitmap path (see Dump in spgemm_kernels.hpp).
+struct DumpBufs {
+    bool on = false;
+    u32 nbm = 0, nch = 0, words = 0;
+    DevArray<uint4> store;
+    DevArray<u32> cnt, list, count, isdense;
+};
+// Allocates the dense-bitmap scratch for the bitmap bin of `h` (nothing when
+// the window is below 2^14 words, the knob disables it or it would not fit)
+// and fills o.dump.
+void setup_dump(clb_controls_s *ctl, const Setup &s, const BinHost &h, DumpBufs &d, Out &o);
+// Expands the dumped rows into cols at the exact row offsets `off`.
+void emit_dense(clb_controls_s *ctl, const u32 *perm, const BinHost &h, DumpBufs &d, const u64 *off, u32 *cols,
+                LaunchTimer *lt, int pass);
+
 BinCfg make_bincfg(const clb_controls_s *ctl, u32 ncols);
 // B row map, kernel arguments and (count == true) per-row product counts.
 void prepare(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &b, u32 row_begin, u32 nrows, Setup &s,
@@ -54,9 +69,10 @@ void bin_rows(clb_controls_s *ctl, const Setup &s, const u32 *key, u32 *nnz_row_
               BinBuf &bb);
 // Reads the binning back (synchronises the stream).
 void read_bins(clb_controls_s *ctl, const BinBuf &bb, BinHost &h);
-// Per-bin sums of nnz_row over the rows of each bin's range of perm (rows
-// marked FAILED count 0); the result is read back into out[NBINS].
-void sum_by_bin(clb_controls_s *ctl, const u32 *perm, const BinHost &h, const u32 *nnz_row, u64 out[NBINS]);
+// Per-bin sums of nnz_row over the rows of each nonempty bin's range of perm
+// (rows marked FAILED count 0); the result is read back into out[NBINS].
+void sum_by_bin(clb_controls_s *ctl, const u32 *perm, const BinHost &h, u32 nrows, const u32 *nnz_row,
+                u64 out[NBINS]);
 // Launches the row kernels of every nonempty bin in `mode` (SYM, NUM or ONE)
 // on the side streams.  family < 0 times launches as sym_* / num_* by mode.
 void launch_bins(clb_controls_s *ctl, const Setup &s, const u32 *perm, const u32 off[NBINS + 1], int mode,

@@ -213,12 +213,27 @@ __device__ __forceinline__ u32 hash_insert(u32 *table, u32 col) {
 constexpr int SYM = 0, NUM = 1, ONE = 2;
 constexpr u32 FAILED = 0xFFFFFFFFu;
 
+// Dense bitmap rows (single-slice windows of 2^14 or 2^15 words) leave the
+// gather kernel as a bitmap in global memory plus popcounts per 1024-word
+// chunk; k_bitmap_emit expands them later into their final positions with
+// coalesced stores.  store == nullptr disables this path.
+constexpr u32 EMIT_CHUNK = 1024;  // words per emission chunk
+struct Dump {
+    uint4 *store;      // [slot][W/4]: slot = the row's index in the bitmap range of perm
+    u32 *chunk_cnt;    // [slot][W / EMIT_CHUNK] popcounts (zero for rows not dumped)
+    u32 *list;         // slots of the dumped rows
+    u32 *count;        // number of dumped rows
+    u32 *isdense;      // [row] 1 if the row was dumped
+    u32 dense_min;     // dump rows with at least this many columns
+};
+
 struct Out {
     u32 *nnz_row;
     const u64 *off;
     u32 *cols;
     const u32 *cap;
     u32 *overflow;  // SYM/NUM: bit flags of inconsistencies; ONE: count of failed rows
+    Dump dump;
 };
 
 template <int MODE>
@@ -396,6 +411,36 @@ __global__ __launch_bounds__(WG) void k_bitmap(Mats g, const u32 *__restrict__ p
             if (d < SPAN) atomicOr(&bm[(u32)(d >> 5)], 1u << (col & 31));
         });
         __syncthreads();
+        if (MODE != SYM && WBITS >= 14 && nslices == 1 && o.dump.store) {
+            // exact count per 1024-word chunk; dense rows are dumped whole
+            constexpr u32 WPT = W / WG, TPC = EMIT_CHUNK / WPT, NCH = W / EMIT_CHUNK;
+            const uint4 *bm4 = reinterpret_cast<const uint4 *>(bm);
+            u32 cnt = 0;
+#pragma unroll
+            for (u32 q = 0; q < WPT / 4; ++q) {
+                const uint4 v = bm4[threadIdx.x * (WPT / 4) + q];
+                cnt += __popc(v.x) + __popc(v.y) + __popc(v.z) + __popc(v.w);
+            }
+            u32 csum = cnt;
+#pragma unroll
+            for (u32 off2 = TPC / 2; off2 > 0; off2 >>= 1) csum += __shfl_xor(csum, off2, TPC);
+            const u32 n = dev::block_sum<u32, WG>(cnt, red);
+            // numeric pass: only a row whose count matches the symbolic pass
+            // (its slot size) is dumped; anything else takes the capped path below
+            if (n >= o.dump.dense_min && (MODE == ONE ? n <= lim : n == lim)) {
+                const u32 slot = blockIdx.x;
+                uint4 *dst = o.dump.store + (u64)slot * (W / 4);
+                for (u32 q = threadIdx.x; q < W / 4; q += WG) dst[q] = bm4[q];
+                if (threadIdx.x % TPC == 0) o.dump.chunk_cnt[(u64)slot * NCH + threadIdx.x / TPC] = csum;
+                if (threadIdx.x == 0) {
+                    o.dump.list[atomicAdd(o.dump.count, 1u)] = slot;
+                    o.dump.isdense[r] = 1;
+                    if (MODE == ONE) o.nnz_row[r] = n;
+                    else if (n != lim) atomicOr(o.overflow, 2u);
+                }
+                return;
+            }
+        }
         if (MODE == SYM) {
             const uint4 *bm4 = reinterpret_cast<const uint4 *>(bm);
             for (u32 t = threadIdx.x; t < W / 4; t += WG) {

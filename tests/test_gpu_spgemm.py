@@ -172,3 +172,44 @@ def test_repeated_calls_reuse_memory(controls):
     ref = oracle.spgemm(A, A)
     for _ in range(5):
         assert_dcsr_equal(controls, cb.dcsr.matrix_multiplication_hash(controls, a, a), ref, "repeat")
+
+
+def _hub_product(seed, nrows=1200, hubs=64, ncols=1 << 20):
+    """A picks 1..30 of `hubs` B rows whose lengths range 300..40000 over
+    2^20 columns: bitmap rows from sparse to dense in one product."""
+    rng = np.random.default_rng(seed)
+    lens = rng.choice([300, 3000, 12000, 40000], hubs)
+    rb = np.repeat(np.arange(hubs, dtype=np.uint32), lens)
+    cbv = rng.integers(0, ncols, rb.size).astype(np.uint32)
+    key = np.unique((rb.astype(np.uint64) << np.uint64(32)) | cbv)
+    rb, cbv = (key >> np.uint64(32)).astype(np.uint32), (key & np.uint64(0xFFFFFFFF)).astype(np.uint32)
+    k = rng.integers(1, 31, nrows)
+    ra = np.repeat(np.arange(nrows, dtype=np.uint32), k)
+    ca = rng.integers(0, hubs, ra.size).astype(np.uint32)
+    key = np.unique((ra.astype(np.uint64) << np.uint64(32)) | ca)
+    ra, ca = (key >> np.uint64(32)).astype(np.uint32), (key & np.uint64(0xFFFFFFFF)).astype(np.uint32)
+    return (ra, ca, nrows, hubs), (rb, cbv, hubs, ncols)
+
+
+@pytest.mark.parametrize("dense_knob", [16, 256, 65536])
+def test_dense_bitmap_rows(controls, dense_knob):
+    """Rows over a 2^20-column window: dumped dense bitmaps (expanded by the
+    emission kernel) and in-LDS emission must give the same exact result."""
+    (ra, ca, m, k), (rb, cbv, k2, n) = _hub_product(5)
+    A, a = pairs_dcsr(controls, ra, ca, m, k)
+    B, b = pairs_dcsr(controls, rb, cbv, k2, n)
+    ref = oracle.spgemm(A, B)
+    old = controls.tuning()
+    t = controls.tuning()
+    t.reserved[1] = dense_knob
+    controls.set_tuning(t)
+    try:
+        got, st = cb.dcsr.spgemm(controls, a, b, timing=True)
+        assert_dcsr_equal(controls, got, ref, f"plain dense_knob={dense_knob}")
+        emits = st["launches"][4 * 1 + 3]
+        assert emits == (0 if dense_knob == 65536 else 1)
+        cb.predictor_reset(controls, 3)
+        got, st = cb.dcsr.spgemm(controls, a, b, mode=cb.SPGEMM_LEARNED, timing=True)
+        assert_dcsr_equal(controls, got, ref, f"learned dense_knob={dense_knob}")
+    finally:
+        controls.set_tuning(old)
