@@ -367,6 +367,27 @@ def _kronecker_product(controls, a: matrix_dcsr, b: matrix_dcsr) -> matrix_dcsr:
     return matrix_dcsr._from_c(m)
 
 
+def _transpose(controls, a: matrix_dcsr) -> matrix_dcsr:
+    """dcsr::transpose (dcsr_transpose.cpp:5-17)."""
+    m, ac = _Dcsr(), a._c()
+    _check(lib().clb_dcsr_transpose(controls.handle, C.byref(ac), C.byref(m)))
+    return matrix_dcsr._from_c(m)
+
+
+def _submatrix(controls, a: matrix_dcsr, i: int, j: int, nrows: int, ncols: int) -> matrix_dcsr:
+    """dcsr::submatrix (dcsr_submatrix.cpp:98-156)."""
+    m, ac = _Dcsr(), a._c()
+    _check(lib().clb_dcsr_submatrix(controls.handle, C.byref(ac), i, j, nrows, ncols, C.byref(m)))
+    return matrix_dcsr._from_c(m)
+
+
+def _reduce(controls, a: matrix_dcsr) -> matrix_dcsr:
+    """dcsr::reduce (dcsr_reduce.cpp:5-31)."""
+    m, ac = _Dcsr(), a._c()
+    _check(lib().clb_dcsr_reduce(controls.handle, C.byref(ac), C.byref(m)))
+    return matrix_dcsr._from_c(m)
+
+
 def _dcsr_addition(controls, a: matrix_dcsr, b: matrix_dcsr) -> matrix_dcsr:
     m, ac, bc = _Dcsr(), a._c(), b._c()
     _check(lib().clb_dcsr_addition(controls.handle, C.byref(ac), C.byref(bc), C.byref(m)))
@@ -393,7 +414,8 @@ def _csr_matrix_addition(controls, a: matrix_csr, b: matrix_csr) -> matrix_csr:
 
 
 dcsr = types.SimpleNamespace(matrix_multiplication_hash=_matrix_multiplication_hash,
-                             kronecker_product=_kronecker_product, spgemm=_spgemm, addition=_dcsr_addition)
+                             kronecker_product=_kronecker_product, spgemm=_spgemm, addition=_dcsr_addition,
+                             transpose=_transpose, submatrix=_submatrix, reduce=_reduce)
 coo = types.SimpleNamespace(matrix_addition=_coo_matrix_addition, sort_arrays=_coo_sort_arrays)
 csr = types.SimpleNamespace(matrix_addition=_csr_matrix_addition)
 
@@ -475,7 +497,8 @@ CLOSURE_PER_TRIPLE = _capi.CLOSURE_PER_TRIPLE
 CLOSURE_BATCHED = _capi.CLOSURE_BATCHED
 
 
-def closure_step(controls, mats: list, triples, mode=CLOSURE_BATCHED, spgemm_mode=SPGEMM_PLAIN) -> dict:
+def closure_step(controls, mats: list, triples, mode=CLOSURE_BATCHED, spgemm_mode=SPGEMM_PLAIN,
+                 block_bound=0) -> dict:
     """X |= Y * Z for every (X, Y, Z) index triple; replaces grown matrices in `mats`."""
     arr = (_Dcsr * len(mats))()
     for k, m in enumerate(mats):
@@ -484,7 +507,7 @@ def closure_step(controls, mats: list, triples, mode=CLOSURE_BATCHED, spgemm_mod
             if b is not None:
                 lib().clb_buffer_retain(b.handle)
     flat = (C.c_uint32 * (3 * len(triples)))(*[int(v) for t in triples for v in t])
-    o = _capi.ClosureOpts(mode, spgemm_mode)
+    o = _capi.ClosureOpts(mode, spgemm_mode, block_bound)
     st = _capi.ClosureStats()
     rc = lib().clb_closure_step(controls.handle, arr, len(mats), flat, len(triples), C.byref(o), C.byref(st))
     for k in range(len(mats)):  # adopt what the struct holds now (old or new matrix)

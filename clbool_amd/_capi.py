@@ -38,7 +38,8 @@ class Csr(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [("hash_max_slots", C.c_uint32), ("bitmap_min_products", C.c_uint32),
                 ("bitmap_slot_ratio", C.c_uint32), ("tiny_max_products", C.c_uint32),
-                ("learned_margin_pct", C.c_uint32), ("reserved", C.c_uint32 * 11)]
+                ("learned_margin_pct", C.c_uint32), ("emit_lane_bits_min", C.c_uint32),
+                ("dump_bits_per_chunk", C.c_uint32), ("reserved", C.c_uint32 * 9)]
 
 
 class SpgemmOpts(C.Structure):
@@ -71,7 +72,8 @@ KERNEL_FAMILIES = [f"{p}_{k}" for p in KERNEL_PASSES for k in KERNEL_KINDS]
 
 
 class ClosureOpts(C.Structure):
-    _fields_ = [("mode", C.c_uint32), ("spgemm_mode", C.c_uint32), ("reserved", C.c_uint32 * 2)]
+    _fields_ = [("mode", C.c_uint32), ("spgemm_mode", C.c_uint32), ("block_bound", C.c_uint32),
+                ("reserved", C.c_uint32)]
 
 
 class ClosureStats(C.Structure):
@@ -141,6 +143,10 @@ SIGNATURES = [
     ("clb_dcsr_spgemm", C.c_int32, [ctl_t, C.POINTER(Dcsr), C.POINTER(Dcsr), C.POINTER(SpgemmOpts),
                                     C.POINTER(Dcsr), C.POINTER(SpgemmStats)]),
     ("clb_dcsr_kronecker", C.c_int32, [ctl_t, C.POINTER(Dcsr), C.POINTER(Dcsr), C.POINTER(Dcsr)]),
+    ("clb_dcsr_transpose", C.c_int32, [ctl_t, C.POINTER(Dcsr), C.POINTER(Dcsr)]),
+    ("clb_dcsr_submatrix", C.c_int32, [ctl_t, C.POINTER(Dcsr), C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                       C.POINTER(Dcsr)]),
+    ("clb_dcsr_reduce", C.c_int32, [ctl_t, C.POINTER(Dcsr), C.POINTER(Dcsr)]),
     ("clb_coo_addition", C.c_int32, [ctl_t, C.POINTER(Coo), C.POINTER(Coo), C.POINTER(Coo)]),
     ("clb_csr_addition", C.c_int32, [ctl_t, C.POINTER(Csr), C.POINTER(Csr), C.POINTER(Csr)]),
     ("clb_dcsr_addition", C.c_int32, [ctl_t, C.POINTER(Dcsr), C.POINTER(Dcsr), C.POINTER(Dcsr)]),

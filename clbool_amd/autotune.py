@@ -17,8 +17,8 @@ import time
 import numpy as np
 
 KNOBS = {
-    "reserved0": [0, 1],                       # in-LDS bitmap emission variant
-    "reserved1": [16, 64, 256, 1024, 65536],   # dense-bitmap dump threshold (65536: off)
+    "emit_lane_bits_min": [256, 1024, 4096],   # 4096: never one lane per bit
+    "dump_bits_per_chunk": [64, 256, 1024, 4096, 65536],   # 65536: no dump
     "bitmap_min_products": [4096, 8192, 16384, 32768, 65536],
     "hash_max_slots": [4096, 8192, 16384],
     "bitmap_slot_ratio": [1, 2, 4],

@@ -8,10 +8,12 @@
 // operands: Y' = [Y_0 | Y_1 | ...] (row block t holds Y_t, its columns shifted
 // by t * n) times Z' = [Z_0; Z_1; ...] (row block t holds Z_t) gives row block
 // t of the result = Y_t * Z_t, so every kernel launch of the SpGEMM covers the
-// rows of all products.  The unions are one sort and deduplication of the
-// keys of all X blocks and all products, followed by a split back into the
-// label matrices.  Every step of both modes is bit-exact: the products and
-// unions are exact set operations, only the launch structure differs.
+// rows of all products.  The product is taken in row blocks of Y' whose
+// output bound (sum over rows of min(products, n)) stays below 2^31, so each
+// block's result fits uint32 indices however large the step grows.  Each X's
+// union is then one sort and deduplication of its own keys and of the product
+// blocks that target it.  Every step of both modes is bit-exact: the products
+// and unions are exact set operations, only the launch structure differs.
 #include "sort.hpp"
 #include "spgemm.hpp"
 
@@ -27,8 +29,6 @@ struct Seg {
     u32 row_shift, col_shift;  // added to row ids and to column ids
     u32 nzr_off, nnz_off;      // position of the block in the stack
 };
-
-constexpr int MAXSEG = 256;
 
 // Copies the blocks' rows (ids shifted) and row pointers (offset by the
 // block's nonzero position) into the stack; one thread per stored row.
@@ -76,21 +76,7 @@ __global__ __launch_bounds__(256) void k_keys_shifted(const u32 *__restrict__ rp
     for (u32 j = rpt[r] + dev::lane(); j < e; j += 64) keys[j] = hi | cols[j];
 }
 
-// First key index of every row block [b*n, (b+1)*n) of sorted unique keys.
-__global__ void k_block_bounds(const u64 *__restrict__ keys, u64 m, u32 n, u32 nblocks, u64 *__restrict__ bound) {
-    const u32 b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b > nblocks) return;
-    const u64 target = (u64)b * n << 32;
-    u64 lo = 0, hi = m;
-    while (lo < hi) {
-        const u64 mid = (lo + hi) / 2;
-        if (keys[mid] < target) lo = mid + 1;
-        else hi = mid;
-    }
-    bound[b] = lo;
-}
-
-// Block b of sorted unique keys -> (rows - b*n, cols) arrays of that block.
+// Sorted unique keys [k0, k0 + cnt) -> (rows - row_shift, cols) arrays.
 __global__ void k_unstack_pairs(const u64 *__restrict__ keys, u64 k0, u32 cnt, u32 row_shift, u32 *__restrict__ rows,
                                 u32 *__restrict__ cols) {
     const u32 e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -98,6 +84,71 @@ __global__ void k_unstack_pairs(const u64 *__restrict__ keys, u64 k0, u32 cnt, u
     const u64 k = keys[k0 + e];
     rows[e] = (u32)(k >> 32) - row_shift;
     cols[e] = (u32)k;
+}
+
+// Keys (row << 32 | col) of stored rows [r0, r1) of a DCSR matrix, rows
+// shifted (modulo 2^32), written from keys[rpt[r] - nnz_base]; one wave per row.
+__global__ __launch_bounds__(256) void k_keys_range(const u32 *__restrict__ rpt, const u32 *__restrict__ rows,
+                                                    const u32 *__restrict__ cols, u32 r0, u32 r1, u32 row_shift,
+                                                    u32 nnz_base, u64 *__restrict__ keys) {
+    const u32 r = r0 + blockIdx.x * 4 + dev::wave_id();
+    if (r >= r1) return;
+    const u64 hi = (u64)(rows[r] + row_shift) << 32;
+    const u32 e = rpt[r + 1];
+    for (u32 j = rpt[r] + dev::lane(); j < e; j += 64) keys[j - nnz_base] = hi | cols[j];
+}
+
+// zlen[row id] = stored length of that row of Z' (zlen pre-filled with 0).
+__global__ void k_row_lengths(const u32 *__restrict__ rpt, const u32 *__restrict__ rows, u32 nzr,
+                              u32 *__restrict__ zlen) {
+    const u32 r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < nzr) zlen[rows[r]] = rpt[r + 1] - rpt[r];
+}
+
+// bound[r] = min(products of stored row r of Y', n): an upper bound of the
+// row's output size.
+__global__ void k_row_bounds(const u32 *__restrict__ rpt, const u32 *__restrict__ cols, u32 nzr,
+                             const u32 *__restrict__ zlen, u32 n, u64 *__restrict__ bound) {
+    const u32 r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nzr) return;
+    u64 p = 0;
+    for (u32 j = rpt[r]; j < rpt[r + 1]; ++j) p += zlen[cols[j]];
+    bound[r] = p < n ? p : n;
+}
+
+// cuts[b] = first stored row whose exclusive bound prefix is >= b * lim.
+__global__ void k_cuts(const u64 *__restrict__ pre, u32 nzr, u64 lim, u32 nb, u32 *__restrict__ cuts) {
+    const u32 b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b > nb) return;
+    if (b == nb) {
+        cuts[b] = nzr;
+        return;
+    }
+    const u64 target = (u64)b * lim;
+    u32 lo = 0, hi = nzr;
+    while (lo < hi) {
+        const u32 mid = lo + (hi - lo) / 2;
+        if (pre[mid] < target) lo = mid + 1;
+        else hi = mid;
+    }
+    cuts[b] = lo;
+}
+
+// For row blocks t = 0..nt of a product (row ids t*n + i): out[2t] = first
+// stored row with id >= t*n, out[2t+1] = its position in cols.
+__global__ void k_row_splits(const u32 *__restrict__ rpt, const u32 *__restrict__ rows, u32 nzr, u32 n, u32 nt,
+                             u32 *__restrict__ out) {
+    const u32 t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t > nt) return;
+    const u64 target = (u64)t * n;
+    u32 lo = 0, hi = nzr;
+    while (lo < hi) {
+        const u32 mid = lo + (hi - lo) / 2;
+        if ((u64)rows[mid] < target) lo = mid + 1;
+        else hi = mid;
+    }
+    out[2 * t] = lo;
+    out[2 * t + 1] = rpt[lo];
 }
 
 // Builds the stack of the given matrices on the device.
@@ -192,7 +243,7 @@ clb_status clb_closure_step(clb_controls ctl, clb_dcsr *mats, uint32_t nmats, co
             }
             for (auto &p : prods) clb_dcsr_release(&p);
         } else {
-            if ((u64)ntriples * n > 0xFFFFFFFFull || (u64)xs.size() * n > 0xFFFFFFFFull)
+            if ((u64)ntriples * n > 0xFFFFFFFFull || n > (1u << 31))
                 raise(INVALID_VALUE, "stacked operands exceed uint32 indices");
             // Y' (row block t = Y_t, columns shifted by t*n) and Z' (row block t = Z_t)
             std::vector<Seg> ys, zs;
@@ -200,93 +251,130 @@ clb_status clb_closure_step(clb_controls ctl, clb_dcsr *mats, uint32_t nmats, co
                 ys.push_back(seg_of(mats[triples[3 * t + 1]], t * n, t * n));
                 zs.push_back(seg_of(mats[triples[3 * t + 2]], t * n, 0));
             }
+            std::vector<clb_dcsr> prods;  // row blocks of Y' * Z'
+            std::vector<std::vector<u32>> splits;
+            auto release_all = [&] {
+                for (auto &p : prods) clb_dcsr_release(&p);
+            };
             clb_dcsr Y = stack(ctl, ys, ntriples * n, ntriples * n);
-            clb_dcsr Z{}, T{};
+            clb_dcsr Z{};
             try {
                 Z = stack(ctl, zs, ntriples * n, n);
-                clb_spgemm_stats ss;
-                spg::spgemm_dispatch(ctl, Y, Z, so, T, ss);
-                st.products = ss.products;
-                st.spgemm_calls = 1;
+                if (Y.nnz && Z.nnz) {
+                    // row blocks of Y' with output bound below 2^31 each
+                    DevArray<u32> zlen(ctl, (u64)ntriples * n);
+                    fill_u32(ctl, zlen.ptr, (u64)ntriples * n, 0u);
+                    k_row_lengths<<<div_up(Z.nzr, 256), 256, 0, ctl->stream>>>(u32p(Z.rpt), u32p(Z.rows), Z.nzr,
+                                                                                zlen.ptr);
+                    CLB_LAUNCHED("row_lengths");
+                    DevArray<u64> pre(ctl, Y.nzr), tot(ctl, 1);
+                    k_row_bounds<<<div_up(Y.nzr, 256), 256, 0, ctl->stream>>>(u32p(Y.rpt), u32p(Y.cols), Y.nzr,
+                                                                               zlen.ptr, n, pre.ptr);
+                    CLB_LAUNCHED("row_bounds");
+                    exclusive_scan_u64(ctl, pre.ptr, pre.ptr, Y.nzr, tot.ptr);
+                    u64 total = 0;
+                    readback(ctl, &total, tot.ptr, 8);
+                    const u64 lim = o.block_bound ? o.block_bound : 1ull << 31;
+                    const u32 nb = (u32)((total + lim - 1) / lim);
+                    DevArray<u32> cuts(ctl, (u64)nb + 1);
+                    k_cuts<<<div_up(nb + 1, 256), 256, 0, ctl->stream>>>(pre.ptr, Y.nzr, lim, nb, cuts.ptr);
+                    CLB_LAUNCHED("cuts");
+                    std::vector<u32> hc(nb + 1);
+                    readback(ctl, hc.data(), cuts.ptr, 4ull * (nb + 1));
+                    DevArray<u32> sp(ctl, 2ull * (ntriples + 1));
+                    for (u32 k = 0; k < nb; ++k) {
+                        if (hc[k + 1] <= hc[k]) continue;
+                        clb_spgemm_opts bo = so;
+                        bo.row_begin = hc[k];
+                        bo.row_end = hc[k + 1];
+                        clb_spgemm_stats ss;
+                        clb_dcsr T{};
+                        spg::spgemm_dispatch(ctl, Y, Z, bo, T, ss);
+                        st.products += ss.products;
+                        ++st.spgemm_calls;
+                        if (!T.nnz) continue;
+                        prods.push_back(T);
+                        // where each product block t starts in T
+                        k_row_splits<<<div_up(ntriples + 1, 256), 256, 0, ctl->stream>>>(
+                            u32p(T.rpt), u32p(T.rows), T.nzr, n, ntriples, sp.ptr);
+                        CLB_LAUNCHED("row_splits");
+                        std::vector<u32> hs(2 * (ntriples + 1));
+                        readback(ctl, hs.data(), sp.ptr, 4ull * hs.size());
+                        splits.push_back(hs);
+                    }
+                }
             } catch (...) {
+                release_all();
                 clb_dcsr_release(&Y);
                 clb_dcsr_release(&Z);
                 throw;
             }
             clb_dcsr_release(&Y);
             clb_dcsr_release(&Z);
-            // keys of every X block (rows shifted by xid*n) and of every product
-            // block (its rows t*n + i re-based onto its X's block)
-            u64 nkeys = T.nnz;
-            for (u32 x : xs) nkeys += mats[x].nnz;
-            DevArray<u64> keys(ctl, nkeys ? nkeys : 1);
-            u64 pos = 0;
             try {
+                // one union per distinct X: its keys and every product block
+                // that targets it, sorted and deduplicated
                 for (u32 k = 0; k < (u32)xs.size(); ++k) {
-                    const clb_dcsr &x = mats[xs[k]];
-                    if (x.nnz)
+                    clb_dcsr &x = mats[xs[k]];
+                    u64 seg = x.nnz;
+                    for (size_t q = 0; q < prods.size(); ++q)
+                        for (u32 t = 0; t < ntriples; ++t)
+                            if (triples[3 * t] == xs[k]) seg += splits[q][2 * t + 3] - splits[q][2 * t + 1];
+                    if (seg == x.nnz) continue;
+                    if (seg >= 0xFFFFFFFFull)
+                        raise(INVALID_VALUE, "a closure union exceeds uint32 nonzeros");
+                    DevArray<u64> keys(ctl, seg);
+                    u64 pos = 0;
+                    if (x.nnz) {
                         k_keys_shifted<<<div_up(x.nzr, 4), 256, 0, ctl->stream>>>(u32p(x.rpt), u32p(x.rows),
-                                                                                   u32p(x.cols), x.nzr, k * n,
-                                                                                   keys.ptr + pos);
-                    pos += x.nnz;
-                }
-                if (T.nnz) {
-                    // T's rows are t*n + i; re-base each product block onto its X block
-                    DevArray<u64> tk(ctl, T.nnz);
-                    k_keys_shifted<<<div_up(T.nzr, 4), 256, 0, ctl->stream>>>(u32p(T.rpt), u32p(T.rows),
-                                                                               u32p(T.cols), T.nzr, 0, tk.ptr);
-                    CLB_LAUNCHED("keys");
-                    // rows of T are sorted; block t occupies keys with row in [t*n, (t+1)*n)
-                    DevArray<u64> tb(ctl, ntriples + 1);
-                    k_block_bounds<<<div_up(ntriples + 1, 256), 256, 0, ctl->stream>>>(tk.ptr, T.nnz, n, ntriples,
-                                                                                        tb.ptr);
-                    std::vector<u64> hb(ntriples + 1);
-                    readback(ctl, hb.data(), tb.ptr, 8 * (ntriples + 1));
-                    for (u32 t = 0; t < ntriples; ++t) {
-                        const u64 cnt = hb[t + 1] - hb[t];
-                        if (!cnt) continue;
-                        // keys (t*n + i) << 32 | c  ->  (xid*n + i) << 32 | c
-                        const i64 delta = ((i64)xid(triples[3 * t]) - (i64)t) * (i64)n;
-                        rebase_keys(ctl, tk.ptr + hb[t], cnt, delta, keys.ptr + pos);
-                        pos += cnt;
+                                                                                   u32p(x.cols), x.nzr, 0, keys.ptr);
+                        CLB_LAUNCHED("keys");
+                        pos = x.nnz;
                     }
+                    for (size_t q = 0; q < prods.size(); ++q) {
+                        const clb_dcsr &T = prods[q];
+                        const std::vector<u32> &hs = splits[q];
+                        for (u32 t = 0; t < ntriples; ++t) {
+                            if (triples[3 * t] != xs[k]) continue;
+                            const u32 r0 = hs[2 * t], r1 = hs[2 * t + 2];
+                            const u32 e0 = hs[2 * t + 1], e1 = hs[2 * t + 3];
+                            if (e1 == e0) continue;
+                            // rows t*n + i -> i
+                            k_keys_range<<<div_up(r1 - r0, 4), 256, 0, ctl->stream>>>(
+                                u32p(T.rpt), u32p(T.rows), u32p(T.cols), r0, r1, 0u - t * n,
+                                (u32)(e0 - (u32)pos), keys.ptr);
+                            CLB_LAUNCHED("keys_range");
+                            pos += e1 - e0;
+                        }
+                    }
+                    sort_pair_keys(ctl, keys.ptr, pos, n - 1, n - 1);
+                    DevArray<u64> uk;
+                    const u32 m = unique_keys(ctl, keys.ptr, pos, uk);
+                    if (m == x.nnz) continue;  // unions only add: equal size means unchanged
+                    clb_dcsr nx{};
+                    nx.nrows = n;
+                    nx.ncols = n;
+                    DevArray<u32> rr(ctl, m);
+                    nx.cols = buffer_new(ctl, 4ull * m);
+                    k_unstack_pairs<<<div_up(m, 256), 256, 0, ctl->stream>>>(uk.ptr, 0, m, 0, rr.ptr,
+                                                                              u32p(nx.cols));
+                    CLB_LAUNCHED("unstack");
+                    try {
+                        rows_to_dcsr(ctl, rr.ptr, m, &nx.rpt, &nx.rows, &nx.nzr);
+                    } catch (...) {
+                        clb_dcsr_release(&nx);
+                        throw;
+                    }
+                    nx.nnz = m;
+                    clb_dcsr_release(&x);
+                    x = nx;
                 }
+                sync(ctl);
             } catch (...) {
-                clb_dcsr_release(&T);
+                release_all();
                 throw;
             }
-            clb_dcsr_release(&T);
-            // one sort + deduplication of all blocks, then split by X block
-            sort_pair_keys(ctl, keys.ptr, pos, (u32)(xs.size() * n - 1), n - 1);
-            clb_buffer_s *ur = nullptr, *uc = nullptr;
-            DevArray<u64> uk;
-            const u32 m = unique_keys(ctl, keys.ptr, pos, uk);
-            DevArray<u64> bnd(ctl, xs.size() + 1);
-            k_block_bounds<<<div_up(xs.size() + 1, 256), 256, 0, ctl->stream>>>(uk.ptr, m, n, (u32)xs.size(),
-                                                                                  bnd.ptr);
-            CLB_LAUNCHED("block_bounds");
-            std::vector<u64> hb(xs.size() + 1);
-            readback(ctl, hb.data(), bnd.ptr, 8 * hb.size());
-            (void)ur;
-            (void)uc;
-            for (u32 k = 0; k < (u32)xs.size(); ++k) {
-                const u64 cnt = hb[k + 1] - hb[k];
-                clb_dcsr &x = mats[xs[k]];
-                if (cnt == x.nnz) continue;  // unions only add: equal size means unchanged
-                clb_dcsr nx{};
-                nx.nrows = n;
-                nx.ncols = n;
-                DevArray<u32> rr(ctl, cnt);
-                nx.cols = buffer_new(ctl, 4 * cnt);
-                k_unstack_pairs<<<div_up(cnt, 256), 256, 0, ctl->stream>>>(uk.ptr, hb[k], (u32)cnt, k * n, rr.ptr,
-                                                                            u32p(nx.cols));
-                CLB_LAUNCHED("unstack");
-                rows_to_dcsr(ctl, rr.ptr, (u32)cnt, &nx.rpt, &nx.rows, &nx.nzr);
-                nx.nnz = (u32)cnt;
-                clb_dcsr_release(&x);
-                x = nx;
-            }
-            sync(ctl);
+            release_all();
         }
         for (u32 k = 0; k < nmats; ++k)
             if (mats[k].nnz != before[k]) {

@@ -176,17 +176,6 @@ u32 unique_keys(clb_controls_s *ctl, const u64 *keys, u64 n, DevArray<u64> &out)
     return m;
 }
 
-__global__ void k_rebase_keys(const u64 *__restrict__ src, u64 n, u64 delta_hi, u64 *__restrict__ dst) {
-    const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) dst[i] = src[i] + delta_hi;  // wraps modulo 2^64 for negative deltas
-}
-
-void rebase_keys(clb_controls_s *ctl, const u64 *src, u64 n, i64 delta, u64 *dst) {
-    if (!n) return;
-    k_rebase_keys<<<div_up(n, 256), 256, 0, ctl->stream>>>(src, n, (u64)delta << 32, dst);
-    CLB_LAUNCHED("rebase_keys");
-}
-
 void pack_pairs(clb_controls_s *ctl, const u32 *rows, const u32 *cols, u64 n, u64 *keys) {
     if (!n) return;
     k_pack_pairs<<<div_up(n, 256), 256, 0, ctl->stream>>>(rows, cols, n, keys);

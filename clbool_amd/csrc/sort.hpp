@@ -18,8 +18,6 @@ u32 unique_keys_to_pairs(clb_controls_s *ctl, const u64 *keys, u64 n, clb_buffer
                          clb_buffer_s **cols_out);
 // Sorted keys -> new array of the distinct keys; returns how many.
 u32 unique_keys(clb_controls_s *ctl, const u64 *keys, u64 n, DevArray<u64> &out);
-// dst[i] = src[i] with delta added to the row half.
-void rebase_keys(clb_controls_s *ctl, const u64 *src, u64 n, i64 delta, u64 *dst);
 void coo_finish(clb_controls_s *ctl, clb_buffer_s *rows, clb_buffer_s *cols, u32 nrows, u32 ncols, u32 nnz,
                 bool sorted, bool nodup, clb_coo &out);
 

@@ -176,7 +176,7 @@ __global__ __launch_bounds__(EMIT_WG) void k_bitmap_emit(const uint4 *__restrict
 void setup_dump(clb_controls_s *ctl, const Setup &s, const BinHost &h, DumpBufs &d, Out &o) {
     o.dump = Dump{};
     d.on = false;
-    const u32 knob = ctl->tuning.reserved[1] ? ctl->tuning.reserved[1] : 256u;
+    const u32 knob = ctl->tuning.dump_bits_per_chunk ? ctl->tuning.dump_bits_per_chunk : 256u;
     if (s.cfg.bm_bits < 14 || knob >= 65536u) return;
     const u64 nslices = ((u64)s.g.ncols + (32ull << s.cfg.bm_bits) - 1) / (32ull << s.cfg.bm_bits);
     d.nbm = h.off[NBINS] - h.off[BIN_BITMAP0];
@@ -254,7 +254,7 @@ void prepare(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &b, u32 row_
     s.g.mapn = s.mapn;
     s.g.row0 = row_begin;
     s.g.ncols = b.ncols;
-    s.g.emit = ctl->tuning.reserved[0] ? EMIT_WORD_LANES : EMIT_LANE_BITS;
+    s.g.emit = ctl->tuning.emit_lane_bits_min;
     s.cfg = make_bincfg(ctl, b.ncols);
     s.prod.reset(ctl, nrows);
     s.scal.reset(ctl, 8);  // [0] total products, [1] nnz total, [2] nzr total, [3] overflow flag

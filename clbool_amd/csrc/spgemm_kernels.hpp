@@ -45,14 +45,16 @@ struct Mats {
     u32 mapn;             // entries of brange
     u32 row0;             // first A stored row of the block
     u32 ncols;            // columns of B (and C)
-    u32 emit;             // bitmap emission variant (EMIT_*)
+    u32 emit;             // bitmap emission: lane-per-bit threshold (EMIT_DEFAULT when 0)
 };
 
-// Bitmap emission variants (tuning knob, picked by the autotuner):
-//  EMIT_LANE_BITS: each lane walks the set bits of its own word.
-//  EMIT_WORD_LANES: 32 lanes share a word, one lane per bit; word values and
-//  word offsets are read from LDS so the unrolled iterations are independent.
-constexpr u32 EMIT_LANE_BITS = 0, EMIT_WORD_LANES = 1;
+// Bitmap emission.  Each wave owns a contiguous range of the bitmap's words
+// and walks it 64 words at a time; a wave scan of popcounts places each
+// word's columns.  A step whose 64 words hold at least `emit` set bits is
+// written one lane per bit (32 lanes per word, two words per iteration:
+// fully coalesced stores); sparser steps are written one lane per word,
+// each lane walking its word's bits.
+constexpr u32 EMIT_DEFAULT = 1024;
 
 struct BinCfg {
     u32 ncols;
@@ -82,14 +84,27 @@ __device__ __forceinline__ u32 classify(u32 P, u32 key, const BinCfg &c) {
 
 // ---------------------------------------------------------------------------
 // Product enumeration of one A row [js, je) by a workgroup of WG threads.
-// `big` is LDS scratch of 2*WG+1 words (unused when WG == 64).  f(col) is
-// called once per product, possibly from divergent lanes.  Every path issues
-// UNROLL independent B-column loads per lane before applying f, so that a
-// wave keeps several gathers in flight (the kernels that use the largest LDS
-// tables run at one workgroup per CU and are otherwise latency bound).
+// `big` is LDS scratch of BIG_WORDS(WG) words, 8-byte aligned (unused when
+// WG == 64).  f(col) is called once per product, possibly from divergent
+// lanes.
+//
+// The A row is taken in chunks of WG nonzeros; each B row of the chunk goes
+// one of three ways by its length:
+//  * >= BIG_ROW (workgroup path): the chunk's long rows are cut into pieces
+//    of 64 columns and listed in LDS with the exclusive prefix of their piece
+//    counts; wave w takes pieces w, w+NW, ... of the concatenation, keeping a
+//    wave-uniform cursor into the list, so a lane's address is the piece base
+//    plus its lane index and rows shorter than a full round leave no wave
+//    idle.  UB pieces per wave are in flight per round (the kernels with the
+//    largest LDS tables run one workgroup per CU; their gather is otherwise
+//    latency bound).
+//  * >= LONG_ROW (wave path): one wave per row, UNROLL loads per lane.
+//  * shorter (lane path): the wave's short rows are concatenated and each
+//    lane finds its row by a binary search over the wave prefix.
 constexpr int UNROLL = 4;
+__host__ __device__ constexpr int BIG_WORDS(int WG) { return 3 * WG + 2 * (WG / 64 + 2); }
 
-template <int WG, typename F>
+template <int WG, int UB, typename F>
 __device__ __forceinline__ void for_each_product(const Mats &g, u32 js, u32 je, u32 *big, F &&f) {
     const int l = dev::lane();
     constexpr int NW = WG / dev::WAVE;
@@ -104,13 +119,23 @@ __device__ __forceinline__ void for_each_product(const Mats &g, u32 js, u32 je, 
                 len = r.y - r.x;
             }
         }
+        // list entry e: big[3e] = first B position, big[3e+1] = end,
+        // big[3e+2] = pieces before it.  Piece counts of distinct B rows sum
+        // to less than 2^32, so the packed scan never carries into the count.
+        u32 nb = 0, np = 0;
         if (NW > 1) {
-            if (threadIdx.x == 0) big[2 * WG] = 0;
-            __syncthreads();
-            if (len >= BIG_ROW) {
-                const u32 s = atomicAdd(&big[2 * WG], 1u);
-                big[2 * s] = bx;
-                big[2 * s + 1] = len;
+            const bool isb = len >= BIG_ROW;
+            u64 *red64 = reinterpret_cast<u64 *>(big + 3 * WG);
+            u64 all;
+            const u64 ex =
+                dev::block_excl_scan<u64, WG>(isb ? ((1ull << 32) | ((len + 63u) >> 6)) : 0ull, red64, all);
+            nb = (u32)(all >> 32);
+            np = (u32)all;
+            if (isb) {
+                const u32 e = (u32)(ex >> 32);
+                big[3 * e] = bx;
+                big[3 * e + 1] = bx + len;
+                big[3 * e + 2] = (u32)ex;
                 len = 0;
             }
         }
@@ -136,9 +161,9 @@ __device__ __forceinline__ void for_each_product(const Mats &g, u32 js, u32 je, 
         // lane expansion of short rows
         const u32 sl = len < LONG_ROW ? len : 0;
         const u32 inc = dev::wave_incl_scan(sl);
-        const u32 tot = __shfl(inc, dev::WAVE - 1, dev::WAVE);
+        const u32 wtot = __shfl(inc, dev::WAVE - 1, dev::WAVE);
         const u32 exc = inc - sl;
-        for (u32 p0 = 0; p0 < tot; p0 += UNROLL * dev::WAVE) {
+        for (u32 p0 = 0; p0 < wtot; p0 += UNROLL * dev::WAVE) {
             u32 c[UNROLL];
 #pragma unroll
             for (int u = 0; u < UNROLL; ++u) {
@@ -154,30 +179,42 @@ __device__ __forceinline__ void for_each_product(const Mats &g, u32 js, u32 je, 
                     }
                 }
                 const u32 sb = __shfl(bx, s, dev::WAVE);
-                c[u] = p < tot ? g.b_cols[sb + (p - es)] : EMPTY;
+                c[u] = p < wtot ? g.b_cols[sb + (p - es)] : EMPTY;
             }
 #pragma unroll
             for (int u = 0; u < UNROLL; ++u)
                 if (c[u] != EMPTY) f(c[u]);
         }
-        if (NW > 1) {
-            __syncthreads();
-            const u32 nb = big[2 * WG];
-            for (u32 e = 0; e < nb; ++e) {
-                const u32 sb = big[2 * e], sl2 = big[2 * e + 1];
-                for (u32 q = threadIdx.x; q < sl2; q += UNROLL * WG) {
-                    u32 c[UNROLL];
+        if (NW > 1 && nb) {
+            __syncthreads();  // the list is complete
+            const u32 w = __builtin_amdgcn_readfirstlane(threadIdx.x / dev::WAVE);
+            // wave-uniform cursor: entry e holds pieces [pre, next)
+            u32 e = 0, pre = 0;
+            u32 next = nb > 1 ? __builtin_amdgcn_readfirstlane(big[5]) : np;
+            u32 eb = __builtin_amdgcn_readfirstlane(big[0]), ee = __builtin_amdgcn_readfirstlane(big[1]);
+            for (u32 q0 = w; q0 < np; q0 += UB * NW) {
+                u32 c[UB];
 #pragma unroll
-                    for (int u = 0; u < UNROLL; ++u) {
-                        const u32 qq = q + u * WG;
-                        c[u] = qq < sl2 ? g.b_cols[sb + qq] : EMPTY;
+                for (int u = 0; u < UB; ++u) {
+                    const u32 q = q0 + u * NW;
+                    c[u] = EMPTY;
+                    if (q < np) {
+                        while (q >= next) {
+                            ++e;
+                            pre = next;
+                            next = e + 1 < nb ? __builtin_amdgcn_readfirstlane(big[3 * e + 5]) : np;
+                            eb = __builtin_amdgcn_readfirstlane(big[3 * e]);
+                            ee = __builtin_amdgcn_readfirstlane(big[3 * e + 1]);
+                        }
+                        const u32 a = eb + ((q - pre) << 6) + (u32)l;
+                        if (a < ee) c[u] = g.b_cols[a];
                     }
-#pragma unroll
-                    for (int u = 0; u < UNROLL; ++u)
-                        if (c[u] != EMPTY) f(c[u]);
                 }
+#pragma unroll
+                for (int u = 0; u < UB; ++u)
+                    if (c[u] != EMPTY) f(c[u]);
             }
-            __syncthreads();
+            __syncthreads();  // before the next chunk rewrites the list
         }
     }
 }
@@ -320,7 +357,7 @@ template <int BITS, int WG, int MODE>
 __global__ __launch_bounds__(WG) void k_hash(Mats g, const u32 *__restrict__ perm, Out o) {
     constexpr u32 T = 1u << BITS;
     __shared__ u32 table[T];
-    __shared__ u32 big[2 * WG + 1];
+    __shared__ __align__(8) u32 big[BIG_WORDS(WG)];
     __shared__ u32 red[WG / 64 + 1];
     const u32 r = perm[blockIdx.x];
     const u32 i = g.row0 + r;
@@ -328,7 +365,7 @@ __global__ __launch_bounds__(WG) void k_hash(Mats g, const u32 *__restrict__ per
     for (u32 t = threadIdx.x; t < T; t += WG) table[t] = EMPTY;
     __syncthreads();
     u32 mine = 0, full = 0;
-    for_each_product<WG>(g, js, je, big, [&](u32 col) {
+    for_each_product<WG, 8>(g, js, je, big, [&](u32 col) {
         const u32 rc = hash_insert<BITS>(table, col);
         mine += rc == 1;
         full |= rc == 2;
@@ -370,15 +407,23 @@ __global__ __launch_bounds__(WG) void k_hash(Mats g, const u32 *__restrict__ per
 #pragma unroll
     for (u32 q = 0; q < PER; ++q)
         if (v[q] != EMPTY) table[pos++] = v[q];
-    const u32 n2 = n <= 1 ? 1 : (1u << (32 - __clz(n - 1)));
-    for (u32 t = n + threadIdx.x; t < n2; t += WG) table[t] = EMPTY;
-    __syncthreads();
-    dev::lds_bitonic_sort<WG>(table, n2);
+    const u32 *sorted = table;
+    if (((n + 63u) & ~63u) <= T / 2) {
+        // the upper half of the table is the merge scratch
+        __syncthreads();
+        sorted = dev::lds_merge_sort<WG>(table, table + T / 2, n);
+    } else {
+        // only when the table overflowed (the passes disagreed)
+        const u32 n2 = n <= 1 ? 1 : (1u << (32 - __clz(n - 1)));
+        for (u32 t = n + threadIdx.x; t < n2; t += WG) table[t] = EMPTY;
+        __syncthreads();
+        dev::lds_bitonic_sort<WG>(table, n2);
+    }
     // never write past the row's slot, even if the passes disagreed
     const u32 w = n < lim ? n : lim;
     if (MODE == NUM && n != lim && threadIdx.x == 0) atomicOr(o.overflow, 2u);
     u32 *out = o.cols + o.off[r];
-    for (u32 t = threadIdx.x; t < w; t += WG) out[t] = table[t];
+    for (u32 t = threadIdx.x; t < w; t += WG) out[t] = sorted[t];
 }
 
 // ---------------------------------------------------------------------------
@@ -389,7 +434,7 @@ __global__ __launch_bounds__(WG) void k_bitmap(Mats g, const u32 *__restrict__ p
     constexpr u32 W = 1u << WBITS;
     constexpr u64 SPAN = 32ull * W;
     __shared__ __align__(16) u32 bm[W];
-    __shared__ u32 big[2 * WG + 1];
+    __shared__ __align__(8) u32 big[BIG_WORDS(WG)];
     __shared__ u32 red[WG / 64 + 1];
     const u32 r = perm[blockIdx.x];
     const u32 i = g.row0 + r;
@@ -406,10 +451,14 @@ __global__ __launch_bounds__(WG) void k_bitmap(Mats g, const u32 *__restrict__ p
             for (u32 t = threadIdx.x; t < W / 4; t += WG) bm4[t] = make_uint4(0, 0, 0, 0);
         }
         __syncthreads();
-        for_each_product<WG>(g, js, je, big, [&](u32 col) {
-            const u64 d = (u64)col - base;
-            if (d < SPAN) atomicOr(&bm[(u32)(d >> 5)], 1u << (col & 31));
-        });
+        if (nslices == 1)
+            for_each_product<WG, 16>(g, js, je, big,
+                                     [&](u32 col) { atomicOr(&bm[col >> 5], 1u << (col & 31)); });
+        else
+            for_each_product<WG, 16>(g, js, je, big, [&](u32 col) {
+                const u64 d = (u64)col - base;
+                if (d < SPAN) atomicOr(&bm[(u32)(d >> 5)], 1u << (col & 31));
+            });
         __syncthreads();
         if (MODE != SYM && WBITS >= 14 && nslices == 1 && o.dump.store) {
             // exact count per 1024-word chunk; dense rows are dumped whole
@@ -448,45 +497,68 @@ __global__ __launch_bounds__(WG) void k_bitmap(Mats g, const u32 *__restrict__ p
                 mine += __popc(v.x) + __popc(v.y) + __popc(v.z) + __popc(v.w);
             }
         } else {
-            // Ordered emission, WG words per round; word offsets within the
-            // round come from a workgroup scan of popcounts.
+            constexpr u32 NW = WG / dev::WAVE, WPW = W / NW;
             const int l = dev::lane();
-            const u32 wbase = dev::wave_id() * dev::WAVE;
-            u32 *offs = big;  // free during emission: holds the round's word offsets
-            for (u32 r0 = 0; r0 < W; r0 += WG) {
-                const u32 word = bm[r0 + threadIdx.x];
-                u32 tot;
-                const u32 p = dev::block_excl_scan<u32, WG>((u32)__popc(word), red, tot);
-                if (g.emit == EMIT_LANE_BITS) {
+            const u32 wv = dev::wave_id();
+            const u32 *wb = bm + wv * WPW;
+            u32 cnt = 0;
+#pragma unroll 4
+            for (u32 q = l; q < WPW; q += dev::WAVE) cnt += __popc(wb[q]);
+            cnt = dev::wave_sum(cnt);
+            // per-wave totals -> wave offsets (red holds NW + 1 words)
+            if (l == 0) red[wv] = cnt;
+            __syncthreads();
+            u32 woff = 0, tot = 0;
+            for (u32 v = 0; v < NW; ++v) {
+                const u32 t = red[v];
+                woff += v < wv ? t : 0u;
+                tot += t;
+            }
+            __syncthreads();
+            if (MODE == ONE && written + tot > lim) {
+                // the capacity was too small: nothing of the row is kept
+                if (threadIdx.x == 0) {
+                    o.nnz_row[r] = FAILED;
+                    atomicAdd(o.overflow, 1u);
+                }
+                return;
+            }
+            u64 pos = written + woff;
+            const u32 hi = (u32)l >> 5, bit = (u32)l & 31u;
+            const u32 below = (1u << bit) - 1u;
+            const u32 dense = g.emit ? g.emit : EMIT_DEFAULT;
+            for (u32 q0 = 0; q0 < WPW; q0 += dev::WAVE) {
+                const u32 word = wb[q0 + l];
+                const u32 pc = (u32)__popc(word);
+                const u32 inc = dev::wave_incl_scan(pc);
+                const u32 st = __shfl(inc, dev::WAVE - 1, dev::WAVE);
+                const u32 exc = inc - pc;
+                const u32 cb = (u32)(base + 32ull * (wv * WPW + q0));
+                if (st >= dense) {
+#pragma unroll 4
+                    for (u32 k = 0; k < 32; ++k) {
+                        const u32 wi = 2 * k + hi;
+                        const u32 v = __shfl(word, (int)wi, dev::WAVE);
+                        const u32 eo = __shfl(exc, (int)wi, dev::WAVE);
+                        if ((v >> bit) & 1u) {
+                            const u64 q = pos + eo + (u32)__popc(v & below);
+                            if (q < lim) out[q] = cb + 32u * wi + bit;
+                        }
+                    }
+                } else if (st) {
                     u32 w = word;
-                    u64 q = written + p;
-                    const u32 colbase = (u32)(base + 32ull * (r0 + threadIdx.x));
+                    u64 q = pos + exc;
+                    const u32 c0 = cb + 32u * (u32)l;
                     while (w) {
-                        const u32 b = __ffs(w) - 1;
-                        if (q < lim) out[q] = colbase + b;
+                        const u32 b2 = __ffs(w) - 1;
+                        if (q < lim) out[q] = c0 + b2;
                         ++q;
                         w &= w - 1;
                     }
-                } else {
-                    offs[threadIdx.x] = p;
-                    __syncthreads();
-                    const u32 hi = (u32)l >> 5, bit = (u32)l & 31u;
-                    const u32 below = (1u << bit) - 1u;
-                    if (__ballot(word != 0)) {
-#pragma unroll 8
-                        for (u32 k = 0; k < 32; ++k) {
-                            const u32 wi = wbase + 2 * k + hi;
-                            const u32 wv = bm[r0 + wi];
-                            if ((wv >> bit) & 1u) {
-                                const u64 q = written + offs[wi] + (u32)__popc(wv & below);
-                                if (q < lim) out[q] = (u32)(base + 32ull * (r0 + wi)) + bit;
-                            }
-                        }
-                    }
-                    __syncthreads();
                 }
-                written += tot;
+                pos += st;
             }
+            written += tot;
         }
         __syncthreads();
     }

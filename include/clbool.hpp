@@ -480,6 +480,22 @@ inline void matrix_addition(Controls &controls, matrix_dcsr &matrix_out, const m
     CLBOOL_CALL(clb_dcsr_addition(controls.get(), &ca, &cbm, &out));
     matrix_out = matrix_dcsr::adopt(out);
 }
+inline void transpose(Controls &controls, matrix_dcsr &matrix_out, const matrix_dcsr &matrix_in) {
+    clb_dcsr ci = matrix_in.c(), out{};
+    CLBOOL_CALL(clb_dcsr_transpose(controls.get(), &ci, &out));
+    matrix_out = matrix_dcsr::adopt(out);
+}
+inline void submatrix(Controls &controls, matrix_dcsr &matrix_out, const matrix_dcsr &matrix_in, uint32_t i,
+                      uint32_t j, uint32_t nrows, uint32_t ncols) {
+    clb_dcsr ci = matrix_in.c(), out{};
+    CLBOOL_CALL(clb_dcsr_submatrix(controls.get(), &ci, i, j, nrows, ncols, &out));
+    matrix_out = matrix_dcsr::adopt(out);
+}
+inline void reduce(Controls &controls, matrix_dcsr &matrix_out, const matrix_dcsr &matrix_in) {
+    clb_dcsr ci = matrix_in.c(), out{};
+    CLBOOL_CALL(clb_dcsr_reduce(controls.get(), &ci, &out));
+    matrix_out = matrix_dcsr::adopt(out);
+}
 }  // namespace dcsr
 
 namespace coo {

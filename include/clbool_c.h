@@ -68,7 +68,15 @@ typedef struct clb_tuning {
     uint32_t tiny_max_products;    /* rows up to this many products use the
                                       wave-level sort kernel                      */
     uint32_t learned_margin_pct;   /* learned sizing: capacity = pred * (100 + m)%  */
-    uint32_t reserved[11];
+    uint32_t emit_lane_bits_min;   /* bitmap emission: 64-word steps with at least
+                                      this many set bits are written one lane per
+                                      bit, sparser ones one lane per word
+                                      (0: 1024)                                   */
+    uint32_t dump_bits_per_chunk;  /* rows over one full 2^20-column window with
+                                      at least this many set bits per 1024 words
+                                      are dumped and expanded by a separate
+                                      emission kernel (0: 256; >= 65536: off)     */
+    uint32_t reserved[9];
 } clb_tuning;
 
 #define CLB_SPGEMM_PLAIN   0u  /* two-pass: symbolic count, scan, numeric      */
@@ -205,6 +213,18 @@ clb_status clb_dcsr_spgemm(clb_controls ctl, const clb_dcsr *a, const clb_dcsr *
                            clb_dcsr *out, clb_spgemm_stats *stats);
 /* dcsr::kronecker_product (src/dcsr/dcsr_kronecker_product.cpp:6-65) */
 clb_status clb_dcsr_kronecker(clb_controls ctl, const clb_dcsr *a, const clb_dcsr *b, clb_dcsr *out);
+/* dcsr::transpose (src/dcsr/dcsr_transpose.cpp:5-17): an empty input gives
+ * an empty (a.ncols x a.nrows) result. */
+clb_status clb_dcsr_transpose(clb_controls ctl, const clb_dcsr *a, clb_dcsr *out);
+/* dcsr::submatrix (src/dcsr/dcsr_submatrix.cpp:98-156): rows [i, i+nrows),
+ * columns [j, j+ncols), shifted to the origin; shape nrows x ncols.  As in the
+ * reference, an empty input or an empty result gives a 0 x 0 matrix. */
+clb_status clb_dcsr_submatrix(clb_controls ctl, const clb_dcsr *a, uint32_t i, uint32_t j, uint32_t nrows,
+                              uint32_t ncols, clb_dcsr *out);
+/* dcsr::reduce (src/dcsr/dcsr_reduce.cpp:5-31): column 0 set in every
+ * nonempty row; shape a.nrows x 1 (the reference labels it a.nzr x 1).  An
+ * empty input gives a 0 x 0 matrix. */
+clb_status clb_dcsr_reduce(clb_controls ctl, const clb_dcsr *a, clb_dcsr *out);
 /* coo::matrix_addition (src/coo/coo_matrix_addition.cpp:32-73) */
 clb_status clb_coo_addition(clb_controls ctl, const clb_coo *a, const clb_coo *b, clb_coo *out);
 /* csr::matrix_addition (src/csr/csr_addition.cpp:26-168) */
@@ -215,12 +235,15 @@ clb_status clb_dcsr_addition(clb_controls ctl, const clb_dcsr *a, const clb_dcsr
 /* ---- closure step of CFPQ/RPQ evaluation (no counterpart in the reference;
  *      its caller, spbla, loops over matrix_multiplication_hash and additions) */
 #define CLB_CLOSURE_PER_TRIPLE 0u  /* one SpGEMM and one union per triple        */
-#define CLB_CLOSURE_BATCHED    1u  /* all products of the step as one SpGEMM on
-                                      block-stacked operands, one batched union */
+#define CLB_CLOSURE_BATCHED    1u  /* all products of the step as SpGEMMs on
+                                      block-stacked operands (one per row block
+                                      of the stack), one union per X           */
 typedef struct clb_closure_opts {
     uint32_t mode;         /* CLB_CLOSURE_*                                   */
     uint32_t spgemm_mode;  /* CLB_SPGEMM_PLAIN or CLB_SPGEMM_LEARNED           */
-    uint32_t reserved[2];
+    uint32_t block_bound;  /* batched: cut the stacked product into row blocks
+                              whose output bound stays below this (0: 2^31)   */
+    uint32_t reserved;
 } clb_closure_opts;
 typedef struct clb_closure_stats {
     uint64_t products;     /* intermediate products of the step               */

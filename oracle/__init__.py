@@ -63,6 +63,12 @@ def lib():
         L.orc_coo_add.argtypes = [u32p, u32p, C.c_uint64, u32p, u32p, C.c_uint64]
         L.orc_kron.restype = C.POINTER(_OrcPairs)
         L.orc_kron.argtypes = [u32p, u32p, C.c_uint64, u32p, u32p, C.c_uint64, C.c_uint32, C.c_uint32]
+        L.orc_transpose.restype = C.POINTER(_OrcMat)
+        L.orc_transpose.argtypes = [u32p, u32p, u32p, C.c_uint32]
+        L.orc_submatrix.restype = C.POINTER(_OrcMat)
+        L.orc_submatrix.argtypes = [u32p, u32p, u32p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]
+        L.orc_reduce.restype = C.POINTER(_OrcMat)
+        L.orc_reduce.argtypes = [u32p, C.c_uint32]
         L.orc_exclusive_scan_u32.restype = C.c_uint64
         L.orc_exclusive_scan_u32.argtypes = [u32p, C.c_uint64]
         L.orc_csr_from_pairs.restype = C.POINTER(_OrcMat)
@@ -257,6 +263,26 @@ def csr_from_pairs(rows, cols, nrows):
     rows, cols = _u32(rows), _u32(cols)
     m = _take_mat(lib().orc_csr_from_pairs(_p32(rows), _p32(cols), rows.size, nrows), with_rows=False)
     return m.rpt, m.cols
+
+
+def transpose(m: Dcsr) -> Dcsr:
+    """matrix_coo_cpu::transpose + coo_to_dcsr_cpu (cpu_matrices.hpp:89-103,
+    coo_utils.cpp:268-294)."""
+    rpt, rows, cols = _u32(m.rpt), _u32(m.rows), _u32(m.cols)
+    return _take_mat(lib().orc_transpose(_p32(rpt), _p32(rows), _p32(cols), m.nzr), m.ncols, m.nrows)
+
+
+def submatrix(m: Dcsr, i, j, nrows, ncols) -> Dcsr:
+    """submatrix_cpu (cpu_matrix_operations.cpp:6-52)."""
+    rpt, rows, cols = _u32(m.rpt), _u32(m.rows), _u32(m.cols)
+    return _take_mat(lib().orc_submatrix(_p32(rpt), _p32(rows), _p32(cols), m.nzr, i, j, nrows, ncols),
+                     nrows, ncols)
+
+
+def reduce(m: Dcsr) -> Dcsr:
+    """reduce (cpu_matrix_operations.cpp:63-76)."""
+    rows = _u32(m.rows)
+    return _take_mat(lib().orc_reduce(_p32(rows), m.nzr), m.nrows, 1)
 
 
 def dcsr_to_csr_rpt(m: Dcsr, nrows):

@@ -495,6 +495,78 @@ orc_pairs *orc_coo_build(const uint32_t *rows, const uint32_t *cols, uint64_t n,
 }
 
 /* ------------------------------------------------------------------------ */
+/* Transpose as the reference's test builds it (test/src/operations.cpp:
+ * 209-238): matrix_coo_cpu::transpose swaps rows and cols and sorts the pairs
+ * (src/core/cpu_matrices.hpp:89-103), then coo_to_dcsr_cpu
+ * (src/coo/coo_utils.cpp:268-294).                                          */
+orc_mat *orc_transpose(const uint32_t *rpt, const uint32_t *rows, const uint32_t *cols, uint32_t nzr) {
+    const uint64_t n = nzr ? rpt[nzr] : 0;
+    if (n == 0) return mat_new();
+    uint64_t *k = (uint64_t *)malloc(n * sizeof(uint64_t));
+    for (uint32_t i = 0; i < nzr; ++i)
+        for (uint32_t q = rpt[i]; q < rpt[i + 1]; ++q) k[q] = ((uint64_t)cols[q] << 32) | rows[i];
+    radix_sort_u64(k, n);
+    orc_pairs *p = pairs_from_keys(k, n);
+    free(k);
+    orc_mat *m = orc_pairs_to_dcsr(p->rows, p->cols, p->n);
+    orc_free_pairs(p);
+    return m;
+}
+
+/* submatrix_cpu: src/common/cpu_matrix_operations.cpp:6-52 (bounds taken in
+ * 64 bits; the reference's i + nrows and j + ncols are uint32 sums). */
+static uint32_t lower_bound32(const uint32_t *a, uint32_t n, uint64_t v) {
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        uint32_t mid = lo + (hi - lo) / 2;
+        if ((uint64_t)a[mid] < v) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+orc_mat *orc_submatrix(const uint32_t *rpt, const uint32_t *rows, const uint32_t *cols, uint32_t nzr, uint32_t i,
+                       uint32_t j, uint32_t nrows, uint32_t ncols) {
+    const uint32_t rb = lower_bound32(rows, nzr, i);
+    const uint32_t re = lower_bound32(rows, nzr, (uint64_t)i + nrows);
+    orc_mat *m = mat_new();
+    if (rb == re) return m; /* :17-20 */
+    vec32 r = {0}, w = {0}, c = {0};
+    uint32_t count = 0;
+    for (uint32_t idx = rb; idx < re; ++idx) {
+        const uint32_t s = rpt[idx], len = rpt[idx + 1] - rpt[idx];
+        const uint32_t lo = lower_bound32(cols + s, len, j);
+        const uint32_t hi = lower_bound32(cols + s, len, (uint64_t)j + ncols);
+        if (hi == lo) continue; /* :36 */
+        for (uint32_t q = lo; q < hi; ++q) v_push(&c, cols[s + q] - j);
+        v_push(&r, count);
+        v_push(&w, rows[idx] - i);
+        count += hi - lo;
+    }
+    v_push(&r, count); /* :46 */
+    m->nzr = (uint32_t)w.n;
+    m->nnz = c.n;
+    m->rpt = r.d;
+    m->rows = w.d;
+    m->cols = c.d;
+    return m;
+}
+
+/* reduce: src/common/cpu_matrix_operations.cpp:63-76 -- column 0 in every
+ * stored row. */
+orc_mat *orc_reduce(const uint32_t *rows, uint32_t nzr) {
+    orc_mat *m = mat_new();
+    m->nzr = nzr;
+    m->nnz = nzr;
+    m->rpt = (uint32_t *)malloc(((uint64_t)nzr + 1) * sizeof(uint32_t));
+    m->rows = (uint32_t *)malloc((nzr ? nzr : 1) * sizeof(uint32_t));
+    m->cols = (uint32_t *)calloc(nzr ? nzr : 1, sizeof(uint32_t));
+    for (uint32_t q = 0; q <= nzr; ++q) m->rpt[q] = q;
+    if (nzr) memcpy(m->rows, rows, nzr * sizeof(uint32_t));
+    return m;
+}
+
+/* ------------------------------------------------------------------------ */
 /* MLP restatement (not in the reference: the learned row-size predictor).  */
 size_t orc_mlp_param_count(orc_mlp_dims d) {
     return (size_t)d.H1 * d.F + d.H1 + (size_t)d.H2 * d.H1 + d.H2 + d.H2 + 1;

@@ -21,6 +21,9 @@
  *   orc_coo_add, orc_kron     set semantics on dense masks
  *   orc_exclusive_scan        closed form (numpy cumsum)
  *   orc_csr_from_pairs        the reference's check_test_utils.cpp:21-39 vector
+ *   orc_transpose             dense transpose; transpose twice is the identity
+ *   orc_submatrix             dense slicing of the brute-force matrix
+ *   orc_reduce                dense row-any
  *   orc_mlp_*                 finite differences (gradients), closed-form Adam
  */
 #ifndef CLBOOL_ORACLE_H
@@ -113,6 +116,14 @@ orc_mat *orc_csr_from_pairs(const uint32_t *rows, const uint32_t *cols, uint64_t
 
 /* dcsr_to_csr: src/common/matrices_conversions.cpp:247-271 (CSR rpt of
  * nrows + 1 built from DCSR rows and lengths; cols shared). */
+/* matrix_coo_cpu::transpose + coo_to_dcsr_cpu (src/core/cpu_matrices.hpp:
+ * 89-103, src/coo/coo_utils.cpp:268-294). */
+orc_mat *orc_transpose(const uint32_t *rpt, const uint32_t *rows, const uint32_t *cols, uint32_t nzr);
+/* submatrix_cpu: src/common/cpu_matrix_operations.cpp:6-52 */
+orc_mat *orc_submatrix(const uint32_t *rpt, const uint32_t *rows, const uint32_t *cols, uint32_t nzr, uint32_t i,
+                       uint32_t j, uint32_t nrows, uint32_t ncols);
+/* reduce: src/common/cpu_matrix_operations.cpp:63-76 */
+orc_mat *orc_reduce(const uint32_t *rows, uint32_t nzr);
 void orc_dcsr_to_csr_rpt(const uint32_t *rpt, const uint32_t *rows, uint32_t nzr, uint32_t nrows, uint32_t *csr_rpt);
 
 /* csr_to_dcsr: src/common/matrices_conversions.cpp:211-244 (drop empty rows).

@@ -33,20 +33,26 @@ def _labels(n_labels, log_n, lo, hi, seed):
     return out, n
 
 
-@pytest.mark.parametrize("mode", [cb.CLOSURE_PER_TRIPLE, cb.CLOSURE_BATCHED])
+@pytest.mark.parametrize("mode,block_bound", [(cb.CLOSURE_PER_TRIPLE, 0), (cb.CLOSURE_BATCHED, 0),
+                                              (cb.CLOSURE_BATCHED, 20000)])
 @pytest.mark.parametrize("spgemm_mode", [cb.SPGEMM_PLAIN, cb.SPGEMM_LEARNED])
-def test_closure_steps_match_oracle(controls, mode, spgemm_mode):
+def test_closure_steps_match_oracle(controls, mode, block_bound, spgemm_mode):
     cpu, n = _labels(12, 10, 1 << 6, 1 << 11, seed=4)
     triples = gen.closure_triples(12, 8, seed=5) + [(0, 1, 2), (0, 3, 4)]  # X repeated
     mats = [cb.matrix_dcsr_from_cpu(controls, m.rpt, m.rows, m.cols, n) for m in cpu]
     if spgemm_mode == cb.SPGEMM_LEARNED:
         cb.predictor_reset(controls, 9)
     for step in range(3):
-        st = cb.closure_step(controls, mats, triples, mode=mode, spgemm_mode=spgemm_mode)
+        st = cb.closure_step(controls, mats, triples, mode=mode, spgemm_mode=spgemm_mode, block_bound=block_bound)
         cpu = _oracle_step(cpu, triples)
         for k in range(len(cpu)):
             assert_dcsr_equal(controls, mats[k], cpu[k], f"step {step} matrix {k}")
-        assert st["spgemm_calls"] == (1 if mode == cb.CLOSURE_BATCHED else len(triples))
+        if mode == cb.CLOSURE_PER_TRIPLE:
+            assert st["spgemm_calls"] == len(triples)
+        elif block_bound == 0:
+            assert st["spgemm_calls"] == 1
+        else:  # the stacked product is cut into several row blocks
+            assert st["spgemm_calls"] > 1
 
 
 def test_closure_reaches_fixed_point(controls):

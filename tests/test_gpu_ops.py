@@ -56,6 +56,72 @@ def test_addition_csr(controls, size, k_a, k_b):
     assert np.array_equal(rpt, erpt) and np.array_equal(cols, ecols)
 
 
+# ---- transpose / submatrix / reduce (test_operations.cpp:10-66,
+# operations.cpp:84-238)
+def _dcsr_of(controls, r, c, m, n):
+    A = oracle.pairs_to_dcsr(r, c, m, n)
+    return A, cb.matrix_dcsr_from_cpu(controls, A.rpt, A.rows, A.cols, m, n)
+
+
+@pytest.mark.parametrize("size,k", [(10, 1), (100, 5), (1000, 10), (5000, 50), (10000, 30)])
+def test_transpose(controls, size, k):
+    r, c = oracle.generate_coo(size * k, size)  # generate_coo_cpu draws in [0, size]
+    A, a = _dcsr_of(controls, r, c, size + 1, size + 1)
+    got = cb.dcsr.transpose(controls, a)
+    assert (got.nrows, got.ncols) == (size + 1, size + 1)
+    assert_dcsr_equal(controls, got, oracle.transpose(A), f"transpose size={size} k={k}")
+
+
+def test_transpose_rectangular_and_empty(controls):
+    r, c = gen.uniform_pairs(20000, 300, 7000, 5)
+    A, a = _dcsr_of(controls, r, c, 300, 7000)
+    got = cb.dcsr.transpose(controls, a)
+    assert (got.nrows, got.ncols) == (7000, 300)
+    assert_dcsr_equal(controls, got, oracle.transpose(A), "transpose 300x7000")
+    e = cb.dcsr.transpose(controls, cb.matrix_dcsr(5, 9))
+    assert (e.nrows, e.ncols, e.nnz) == (9, 5, 0)
+
+
+@pytest.mark.parametrize("size,k", [(100, 10), (1000, 10), (3000, 50), (10000, 5)])
+def test_submatrix(controls, size, k):
+    ar, ac = oracle.generate_coo_pairs(size * k, size)
+    A, a = _dcsr_of(controls, ar, ac, size, size)
+    rng = np.random.default_rng(size + k)
+    for trial in range(4):
+        i1, i2, i3, i4 = (int(x) for x in rng.integers(0, size + 1, 4))
+        i, nrows, j, ncols = min(i1, i2), abs(i1 - i2), min(i3, i4), abs(i3 - i4)
+        want = oracle.submatrix(A, i, j, nrows, ncols)
+        got = cb.dcsr.submatrix(controls, a, i, j, nrows, ncols)
+        if want.nnz == 0:
+            assert (got.nnz, got.nrows, got.ncols) == (0, 0, 0)
+        else:
+            assert (got.nrows, got.ncols) == (nrows, ncols)
+            assert_dcsr_equal(controls, got, want, f"submatrix {i},{j},{nrows},{ncols}")
+
+
+def test_submatrix_zero_and_edges(controls):
+    ar, ac = oracle.generate_coo_pairs(5000, 500)
+    A, a = _dcsr_of(controls, ar, ac, 500, 500)
+    for i, j, nrows, ncols in [(0, 0, 0, 0), (0, 0, 500, 500), (499, 499, 1, 1), (0, 250, 500, 0),
+                               (100, 100, 2 ** 32 - 101, 2 ** 32 - 101)]:
+        want = oracle.submatrix(A, i, j, nrows, ncols)
+        got = cb.dcsr.submatrix(controls, a, i, j, nrows, ncols)
+        if want.nnz == 0:
+            assert (got.nnz, got.nrows, got.ncols) == (0, 0, 0)
+        else:
+            assert_dcsr_equal(controls, got, want, f"submatrix {i},{j},{nrows},{ncols}")
+
+
+@pytest.mark.parametrize("size,k", [(100, 1), (1000, 10), (30000, 3)])
+def test_reduce(controls, size, k):
+    r, c = oracle.generate_coo(size * k, size)
+    A, a = _dcsr_of(controls, r, c, size + 1, size + 1)
+    got = cb.dcsr.reduce(controls, a)
+    assert (got.nrows, got.ncols) == (size + 1, 1)
+    assert_dcsr_equal(controls, got, oracle.reduce(A), f"reduce size={size}")
+    assert cb.dcsr.reduce(controls, cb.matrix_dcsr(4, 4)).nnz == 0
+
+
 # ---- DCSR union (the closure's X += T)
 def test_addition_dcsr(controls):
     r1, c1 = gen.rmat(12, 8, 1)

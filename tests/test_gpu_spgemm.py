@@ -201,7 +201,7 @@ def test_dense_bitmap_rows(controls, dense_knob):
     ref = oracle.spgemm(A, B)
     old = controls.tuning()
     t = controls.tuning()
-    t.reserved[1] = dense_knob
+    t.dump_bits_per_chunk = dense_knob
     controls.set_tuning(t)
     try:
         got, st = cb.dcsr.spgemm(controls, a, b, timing=True)

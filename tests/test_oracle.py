@@ -208,6 +208,41 @@ def test_kron_matches_numpy():
     assert np.array_equal(r, er) and np.array_equal(c, ec)
 
 
+def test_transpose_matches_dense():
+    rng = np.random.default_rng(12)
+    d = random_dense(rng, 70, 45, 0.05)
+    A = dcsr_from_dense(d)
+    T = oracle.transpose(A)
+    assert (T.nrows, T.ncols) == (45, 70)
+    assert np.array_equal(dense_from_dcsr(T, 45, 70), d.T)
+    back = oracle.transpose(T)
+    assert np.array_equal(back.rpt, A.rpt) and np.array_equal(back.rows, A.rows) and np.array_equal(back.cols, A.cols)
+    # a planted mistake (no transpose) fails the dense check
+    assert not np.array_equal(dense_from_dcsr(A, 70, 70)[:45, :70], d.T)
+    assert oracle.transpose(dcsr_from_dense(np.zeros((3, 4), np.uint8))).nnz == 0
+
+
+@pytest.mark.parametrize("i,j,nrows,ncols", [(0, 0, 60, 50), (10, 5, 20, 30), (59, 49, 1, 1), (0, 0, 0, 0),
+                                             (30, 0, 0, 10), (5, 7, 100, 100), (60, 0, 5, 5)])
+def test_submatrix_matches_dense_slice(i, j, nrows, ncols):
+    rng = np.random.default_rng(13)
+    d = random_dense(rng, 60, 50, 0.08)
+    S = oracle.submatrix(dcsr_from_dense(d), i, j, nrows, ncols)
+    want = d[i:i + nrows, j:j + ncols]
+    got = dense_from_dcsr(S, want.shape[0], want.shape[1]) if S.nnz else np.zeros_like(want)
+    assert np.array_equal(got, want)
+    # canonical: no empty rows, rows ascending
+    assert np.all(np.diff(S.rpt.astype(np.int64)) > 0) and np.all(np.diff(S.rows.astype(np.int64)) > 0)
+
+
+def test_reduce_is_row_any():
+    rng = np.random.default_rng(14)
+    d = random_dense(rng, 80, 30, 0.02)
+    R = oracle.reduce(dcsr_from_dense(d))
+    assert np.array_equal(dense_from_dcsr(R, 80, 1)[:, 0], d.any(axis=1).astype(np.uint8))
+    assert np.all(R.cols == 0)
+
+
 def test_exclusive_scan_closed_form():
     rng = np.random.default_rng(6)
     a = rng.integers(0, 4, 100001).astype(np.uint32)
