@@ -122,6 +122,76 @@ __device__ void lds_bitonic_sort(uint32_t *keys, uint32_t n2) {
     }
 }
 
+// DPP move within rows of 16 lanes (GFX9 encoding: row_shl:n = 0x100 + n,
+// row_shr:n = 0x110 + n); lanes whose source lies outside their row get `old`.
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp_row(uint32_t old, uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)v, CTRL, 0xF, 0xF, false);
+}
+
+// Inclusive sum across a wave with DPP moves only (no LDS traffic): shifts
+// 1, 2, 4, 8 within rows of 16 lanes, then row_bcast:15 (0x142, into rows 1
+// and 3) and row_bcast:31 (0x143, into rows 2 and 3).  All 64 lanes must be
+// active.
+__device__ __forceinline__ uint32_t wave_incl_sum_dpp(uint32_t v) {
+    v += dpp_row<0x111>(0u, v);
+    v += dpp_row<0x112>(0u, v);
+    v += dpp_row<0x114>(0u, v);
+    v += dpp_row<0x118>(0u, v);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xA, 0xF, false);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xC, 0xF, false);
+    return v;
+}
+
+// Number of set bits of m in the lanes below this one.
+__device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// One step of ordered bitmap emission by a wave: lane l holds bitmap word
+// `word` (64 consecutive words, the first one's columns start at cb).  The
+// set bits are written as columns, in ascending order, to out[pos ...];
+// positions at or beyond lim are dropped.  A step with at least `dense` set
+// bits is written one lane per bit: two words per iteration, a ballot places
+// each set lane, every store instruction is one contiguous run.  Sparser
+// steps are written one lane per word, each lane walking its word's bits from
+// its prefix position.  Returns the step's number of set bits.  All 64 lanes
+// must be active.
+__device__ __forceinline__ uint32_t emit_step(uint32_t word, uint32_t cb, uint64_t pos, uint32_t *out,
+                                              uint64_t lim, uint32_t dense) {
+    const uint32_t l = (uint32_t)lane();
+    const uint32_t pc = (uint32_t)__popc(word);
+    const uint32_t inc = wave_incl_sum_dpp(pc);
+    const uint32_t st = (uint32_t)__builtin_amdgcn_readlane((int)inc, 63);
+    if (st == 0) return 0;
+    if (st >= dense) {
+        uint64_t p = pos;
+        const uint32_t sh = l & 31u;
+        for (uint32_t k = 0; k < 32; ++k) {
+            const uint32_t w0 = (uint32_t)__builtin_amdgcn_readlane((int)word, (int)(2 * k));
+            const uint32_t w1 = (uint32_t)__builtin_amdgcn_readlane((int)word, (int)(2 * k + 1));
+            if ((w0 | w1) == 0) continue;
+            const uint32_t v = l < 32 ? w0 : w1;
+            const bool set = (v >> sh) & 1u;
+            const uint64_t m = __ballot(set);
+            const uint64_t q = p + lanes_below(m);
+            if (set && q < lim) out[q] = cb + 64u * k + l;
+            p += (uint64_t)__popcll(m);
+        }
+    } else {
+        uint32_t w = word;
+        uint64_t q = pos + (inc - pc);
+        const uint32_t c0 = cb + 32u * l;
+        while (w) {
+            const uint32_t b = (uint32_t)__ffs(w) - 1u;
+            if (q < lim) out[q] = c0 + b;
+            ++q;
+            w &= w - 1u;
+        }
+    }
+    return st;
+}
+
 // Ascending bitonic sort of one value per lane across a wave (64 values).
 __device__ __forceinline__ uint32_t wave_sort64(uint32_t v) {
     const int l = lane();

@@ -131,45 +131,41 @@ __global__ void k_set_positions(const u32 *__restrict__ nnz_row, const u64 *__re
 }
 
 // Expands one 1024-word chunk of a dumped bitmap row into its columns at
-// their final positions: four rounds of 256 words, each staged in LDS so that
-// the global stores of a round are one contiguous run.
+// their final positions: each of the 4 waves owns 256 words, places them by
+// the waves' popcounts, and writes them 64 words at a time (dev::emit_step).
 // gpre = exclusive prefix of chunk_cnt over all slots.
 constexpr int EMIT_WG = 256;
 __global__ __launch_bounds__(EMIT_WG) void k_bitmap_emit(const uint4 *__restrict__ store, u32 nch,
                                                          const u32 *__restrict__ chunk_cnt,
                                                          const u64 *__restrict__ gpre, const u32 *__restrict__ list,
                                                          const u32 *__restrict__ perm, const u64 *__restrict__ off,
-                                                         u32 words_per_row, u32 *__restrict__ cols) {
-    __shared__ u32 stage[EMIT_WG * 32];
-    __shared__ u32 red[EMIT_WG / 64 + 1];
+                                                         u32 words_per_row, u32 dense, u32 *__restrict__ cols) {
+    constexpr u32 NW = EMIT_WG / 64, WPW = EMIT_CHUNK / NW, STEPS = WPW / 64;
+    __shared__ u32 red[NW];
     const u32 li = blockIdx.x / nch, c = blockIdx.x % nch;
     const u32 slot = list[li];
     const u64 ci = (u64)slot * nch + c;
     if (chunk_cnt[ci] == 0) return;
     const u32 r = perm[slot];
-    u32 *out = cols + off[r] + (gpre[ci] - gpre[(u64)slot * nch]);
-    const u32 *words = reinterpret_cast<const u32 *>(store) + (u64)slot * words_per_row + (u64)c * EMIT_CHUNK;
-    const u32 colbase = c * EMIT_CHUNK * 32;
-    u32 w[EMIT_CHUNK / EMIT_WG];
+    u32 *out = cols + off[r];
+    const u64 lim = off[r + 1] - off[r];
+    const u32 wv = dev::wave_id(), l = (u32)dev::lane();
+    const u32 *words = reinterpret_cast<const u32 *>(store) + (u64)slot * words_per_row + (u64)c * EMIT_CHUNK + wv * WPW;
+    u32 w[STEPS];
+    u32 cnt = 0;
 #pragma unroll
-    for (int k = 0; k < (int)(EMIT_CHUNK / EMIT_WG); ++k) w[k] = words[k * EMIT_WG + threadIdx.x];
-    u32 done = 0;
-#pragma unroll
-    for (int k = 0; k < (int)(EMIT_CHUNK / EMIT_WG); ++k) {
-        u32 tot;
-        u32 p = dev::block_excl_scan<u32, EMIT_WG>((u32)__popc(w[k]), red, tot);
-        u32 x = w[k];
-        const u32 cb = colbase + 32u * (k * EMIT_WG + threadIdx.x);
-        while (x) {
-            const u32 b = __ffs(x) - 1;
-            stage[p++] = cb + b;
-            x &= x - 1;
-        }
-        __syncthreads();
-        for (u32 t = threadIdx.x; t < tot; t += EMIT_WG) out[done + t] = stage[t];
-        done += tot;
-        __syncthreads();
+    for (u32 k = 0; k < STEPS; ++k) {
+        w[k] = words[k * 64 + l];
+        cnt += __popc(w[k]);
     }
+    cnt = dev::wave_sum(cnt);
+    if (l == 0) red[wv] = cnt;
+    __syncthreads();
+    u64 p = gpre[ci] - gpre[(u64)slot * nch];
+    for (u32 v = 0; v < wv; ++v) p += red[v];
+    const u32 colbase = (c * EMIT_CHUNK + wv * WPW) * 32u;
+#pragma unroll
+    for (u32 k = 0; k < STEPS; ++k) p += dev::emit_step(w[k], colbase + k * 64u * 32u, p, out, lim, dense);
 }
 
 // ---------------------------------------------------------------------------
@@ -211,9 +207,10 @@ void emit_dense(clb_controls_s *ctl, const u32 *perm, const BinHost &h, DumpBufs
     DevArray<u64> gpre(ctl, d.cnt.n + 1);
     exclusive_scan_u32_to_u64(ctl, d.cnt.ptr, gpre.ptr, d.cnt.n, nullptr);
     if (lt) lt->begin(ctl->stream);
+    const u32 dense = ctl->tuning.emit_lane_bits_min ? ctl->tuning.emit_lane_bits_min : EMIT_DEFAULT;
     k_bitmap_emit<<<(u64)count * d.nch, EMIT_WG, 0, ctl->stream>>>(d.store.ptr, d.nch, d.cnt.ptr, gpre.ptr,
                                                                     d.list.ptr, perm + h.off[BIN_BITMAP0], off,
-                                                                    d.words, cols);
+                                                                    d.words, dense, cols);
     CLB_LAUNCHED("bitmap_emit");
     if (lt) lt->end((int)CLB_KF(pass, CLB_KK_EMIT), ctl->stream);
 }
@@ -375,6 +372,11 @@ LaunchTimer::~LaunchTimer() {
 
 // Launch the per-bin kernels of one pass.  off[b]..off[b+1] is bin b's range
 // of perm.  numeric == false: write nnz_row; true: write columns.
+template <int WBITS, int WG, int MODE>
+static void launch_bitmap(hipStream_t st, const Mats &g, const u32 *perm, u32 n, const Out &o) {
+    k_bitmap<WBITS, WG, MODE><<<n, WG, 0, st>>>(g, perm, o);
+}
+
 void launch_bins(clb_controls_s *ctl, const Setup &s, const u32 *perm, const u32 off[NBINS + 1], int mode,
                  const Out &o, LaunchTimer *lt, int family) {
     fork_side(ctl);
@@ -433,18 +435,25 @@ void launch_bins(clb_controls_s *ctl, const Setup &s, const u32 *perm, const u32
             const u32 *pp = perm + off[BIN_BITMAP0];
             hipStream_t st = next();
             t.begin(st);
+#define BITMAP3(WB, WGS)                                                   \
+    do {                                                                   \
+        if (mode == SYM) launch_bitmap<WB, WGS, SYM>(st, s.g, pp, n, o); \
+        else if (mode == NUM) launch_bitmap<WB, WGS, NUM>(st, s.g, pp, n, o); \
+        else launch_bitmap<WB, WGS, ONE>(st, s.g, pp, n, o);          \
+    } while (0)
             switch (s.cfg.bm_bits) {
-                case 6: DISPATCH3(k_bitmap, n, 64, st, 6, 64); break;
-                case 7: DISPATCH3(k_bitmap, n, 64, st, 7, 64); break;
-                case 8: DISPATCH3(k_bitmap, n, 64, st, 8, 64); break;
-                case 9: DISPATCH3(k_bitmap, n, 256, st, 9, 256); break;
-                case 10: DISPATCH3(k_bitmap, n, 256, st, 10, 256); break;
-                case 11: DISPATCH3(k_bitmap, n, 256, st, 11, 256); break;
-                case 12: DISPATCH3(k_bitmap, n, 512, st, 12, 512); break;
-                case 13: DISPATCH3(k_bitmap, n, 512, st, 13, 512); break;
-                case 14: DISPATCH3(k_bitmap, n, 1024, st, 14, 1024); break;
-                case 15: DISPATCH3(k_bitmap, n, 1024, st, 15, 1024); break;
+                case 6: BITMAP3(6, 64); break;
+                case 7: BITMAP3(7, 64); break;
+                case 8: BITMAP3(8, 64); break;
+                case 9: BITMAP3(9, 256); break;
+                case 10: BITMAP3(10, 256); break;
+                case 11: BITMAP3(11, 256); break;
+                case 12: BITMAP3(12, 512); break;
+                case 13: BITMAP3(13, 512); break;
+                case 14: BITMAP3(14, 1024); break;
+                case 15: BITMAP3(15, 1024); break;
             }
+#undef BITMAP3
             CLB_LAUNCHED("bitmap");
             t.end(fam(2), st);
         }
@@ -493,7 +502,10 @@ void assemble(clb_controls_s *ctl, const Setup &s, const clb_dcsr &a, const u32 
 void scan_sizes(clb_controls_s *ctl, Setup &s, const u32 *nnz_row, DevArray<u64> &off, DevArray<u32> &pos) {
     off.reset(ctl, s.nrows + 1);
     pos.reset(ctl, s.nrows + 1);
-    exclusive_scan_u32_to_u64(ctl, nnz_row, off.ptr, s.nrows, s.scal.ptr + 1);
+    // off[nrows] holds the total too (row r's size is off[r + 1] - off[r])
+    exclusive_scan_u32_to_u64(ctl, nnz_row, off.ptr, s.nrows, off.ptr + s.nrows);
+    CLB_HIP(hipMemcpyAsync(s.scal.ptr + 1, off.ptr + s.nrows, 8, hipMemcpyDeviceToDevice, ctl->stream),
+            COPY_BUFFER_ERROR);
     k_nonzero_flags<<<div_up(s.nrows, 256), 256, 0, ctl->stream>>>(nnz_row, s.nrows, pos.ptr);
     CLB_LAUNCHED("nonzero_flags");
     exclusive_scan_u32(ctl, pos.ptr, pos.ptr, s.nrows, reinterpret_cast<u32 *>(s.scal.ptr + 2));

@@ -49,11 +49,9 @@ struct Mats {
 };
 
 // Bitmap emission.  Each wave owns a contiguous range of the bitmap's words
-// and walks it 64 words at a time; a wave scan of popcounts places each
-// word's columns.  A step whose 64 words hold at least `emit` set bits is
-// written one lane per bit (32 lanes per word, two words per iteration:
-// fully coalesced stores); sparser steps are written one lane per word,
-// each lane walking its word's bits.
+// and walks it 64 words at a time (dev::emit_step): a step whose 64 words
+// hold at least `emit` set bits is written one lane per bit with coalesced
+// stores, sparser steps one lane per word.
 constexpr u32 EMIT_DEFAULT = 1024;
 
 struct BinCfg {
@@ -90,24 +88,27 @@ __device__ __forceinline__ u32 classify(u32 P, u32 key, const BinCfg &c) {
 //
 // The A row is taken in chunks of WG nonzeros; each B row of the chunk goes
 // one of three ways by its length:
-//  * >= BIG_ROW (workgroup path): the chunk's long rows are cut into pieces
-//    of 64 columns and listed in LDS with the exclusive prefix of their piece
-//    counts; wave w takes pieces w, w+NW, ... of the concatenation, keeping a
-//    wave-uniform cursor into the list, so a lane's address is the piece base
-//    plus its lane index and rows shorter than a full round leave no wave
-//    idle.  UB pieces per wave are in flight per round (the kernels with the
-//    largest LDS tables run one workgroup per CU; their gather is otherwise
-//    latency bound).
+//  * >= BIGMIN (workgroup path): the chunk's long rows are cut into windows
+//    of UB * 64 columns and listed in LDS with the exclusive prefix of their
+//    window counts; wave w takes windows w, w+NW, ... of the concatenation.
+//    A full window is UB loads of 64 consecutive columns at one base address
+//    (immediate offsets, no per-lane checks); only a row's last window needs
+//    clamping.  The wave keeps a cursor into the list, with 64 list entries
+//    cached one per lane (read with readlane), so moving to the next B row
+//    needs no LDS access.  Windows are double-buffered: the loads of one are
+//    in flight while the previous one's columns are applied (the kernels
+//    with the largest LDS tables run one workgroup per CU).
 //  * >= LONG_ROW (wave path): one wave per row, UNROLL loads per lane.
 //  * shorter (lane path): the wave's short rows are concatenated and each
 //    lane finds its row by a binary search over the wave prefix.
 constexpr int UNROLL = 4;
 __host__ __device__ constexpr int BIG_WORDS(int WG) { return 3 * WG + 2 * (WG / 64 + 2); }
 
-template <int WG, int UB, typename F>
+template <int WG, int UB, u32 BIGMIN, typename F>
 __device__ __forceinline__ void for_each_product(const Mats &g, u32 js, u32 je, u32 *big, F &&f) {
     const int l = dev::lane();
     constexpr int NW = WG / dev::WAVE;
+    constexpr u32 WIN = UB * 64;
     for (u32 base = js; base < je; base += WG) {
         const u32 j = base + threadIdx.x;
         u32 bx = 0, len = 0;
@@ -120,17 +121,18 @@ __device__ __forceinline__ void for_each_product(const Mats &g, u32 js, u32 je, 
             }
         }
         // list entry e: big[3e] = first B position, big[3e+1] = end,
-        // big[3e+2] = pieces before it.  Piece counts of distinct B rows sum
-        // to less than 2^32, so the packed scan never carries into the count.
-        u32 nb = 0, np = 0;
+        // big[3e+2] = windows before it.  Window counts of distinct B rows
+        // sum to less than 2^32, so the packed scan never carries into the
+        // count.
+        u32 nb = 0, nwin = 0;
         if (NW > 1) {
-            const bool isb = len >= BIG_ROW;
+            const bool isb = len >= BIGMIN;
             u64 *red64 = reinterpret_cast<u64 *>(big + 3 * WG);
             u64 all;
             const u64 ex =
-                dev::block_excl_scan<u64, WG>(isb ? ((1ull << 32) | ((len + 63u) >> 6)) : 0ull, red64, all);
+                dev::block_excl_scan<u64, WG>(isb ? ((1ull << 32) | ((len + WIN - 1) / WIN)) : 0ull, red64, all);
             nb = (u32)(all >> 32);
-            np = (u32)all;
+            nwin = (u32)all;
             if (isb) {
                 const u32 e = (u32)(ex >> 32);
                 big[3 * e] = bx;
@@ -188,31 +190,67 @@ __device__ __forceinline__ void for_each_product(const Mats &g, u32 js, u32 je, 
         if (NW > 1 && nb) {
             __syncthreads();  // the list is complete
             const u32 w = __builtin_amdgcn_readfirstlane(threadIdx.x / dev::WAVE);
-            // wave-uniform cursor: entry e holds pieces [pre, next)
+            // list entries [cbase, cbase + 64) cached one per lane
+            u32 cbase = 0, cbx = 0, cend = 0, cpre = 0;
+            auto load_cache = [&](u32 b0) {
+                cbase = b0;
+                const u32 q = b0 + (u32)l;
+                cbx = q < nb ? big[3 * q] : 0u;
+                cend = q < nb ? big[3 * q + 1] : 0u;
+                cpre = q < nb ? big[3 * q + 2] : nwin;
+            };
+            auto lane_val = [](u32 v, u32 lane) { return (u32)__builtin_amdgcn_readlane((int)v, (int)lane); };
+            load_cache(0);
+            // wave-uniform cursor: entry e holds windows [pre, next)
             u32 e = 0, pre = 0;
-            u32 next = nb > 1 ? __builtin_amdgcn_readfirstlane(big[5]) : np;
-            u32 eb = __builtin_amdgcn_readfirstlane(big[0]), ee = __builtin_amdgcn_readfirstlane(big[1]);
-            for (u32 q0 = w; q0 < np; q0 += UB * NW) {
-                u32 c[UB];
+            u32 eb = lane_val(cbx, 0), ee = lane_val(cend, 0), next = lane_val(cpre, 1);
+            // loads window k into c; returns its number of columns (0: none)
+            auto fetch = [&](u32 k, u32(&c)[UB]) -> u32 {
+                if (k >= nwin) return 0u;
+                while (k >= next) {
+                    ++e;
+                    if (e + 1 - cbase >= (u32)dev::WAVE) load_cache(e);
+                    const u32 s = e - cbase;
+                    pre = next;
+                    eb = lane_val(cbx, s);
+                    ee = lane_val(cend, s);
+                    next = lane_val(cpre, s + 1);
+                }
+                const u32 start = eb + (k - pre) * WIN;
+                const u32 n = ee - start < WIN ? ee - start : WIN;
+                const u32 *src = g.b_cols + start;
+                if (n == WIN) {
 #pragma unroll
-                for (int u = 0; u < UB; ++u) {
-                    const u32 q = q0 + u * NW;
-                    c[u] = EMPTY;
-                    if (q < np) {
-                        while (q >= next) {
-                            ++e;
-                            pre = next;
-                            next = e + 1 < nb ? __builtin_amdgcn_readfirstlane(big[3 * e + 5]) : np;
-                            eb = __builtin_amdgcn_readfirstlane(big[3 * e]);
-                            ee = __builtin_amdgcn_readfirstlane(big[3 * e + 1]);
-                        }
-                        const u32 a = eb + ((q - pre) << 6) + (u32)l;
-                        if (a < ee) c[u] = g.b_cols[a];
+                    for (int u = 0; u < UB; ++u) c[u] = src[u * 64 + l];
+                } else {
+                    // the row's last window: lanes past its end re-read its
+                    // last column and are masked off when applied
+#pragma unroll
+                    for (int u = 0; u < UB; ++u) {
+                        const u32 o = (u32)(u * 64 + l);
+                        c[u] = src[o < n ? o : n - 1];
                     }
                 }
+                return n;
+            };
+            auto apply = [&](const u32(&c)[UB], u32 n) {
+                if (n == WIN) {
 #pragma unroll
-                for (int u = 0; u < UB; ++u)
-                    if (c[u] != EMPTY) f(c[u]);
+                    for (int u = 0; u < UB; ++u) f(c[u]);
+                } else {
+#pragma unroll
+                    for (int u = 0; u < UB; ++u)
+                        if ((u32)(u * 64 + l) < n) f(c[u]);
+                }
+            };
+            u32 ca[UB], cb[UB];
+            u32 na = fetch(w, ca), nb2 = 0;
+            for (u32 k = w; k < nwin; k += 2 * NW) {
+                nb2 = fetch(k + NW, cb);
+                apply(ca, na);
+                if (k + NW >= nwin) break;
+                na = fetch(k + 2 * NW, ca);
+                apply(cb, nb2);
             }
             __syncthreads();  // before the next chunk rewrites the list
         }
@@ -365,11 +403,12 @@ __global__ __launch_bounds__(WG) void k_hash(Mats g, const u32 *__restrict__ per
     for (u32 t = threadIdx.x; t < T; t += WG) table[t] = EMPTY;
     __syncthreads();
     u32 mine = 0, full = 0;
-    for_each_product<WG, 8>(g, js, je, big, [&](u32 col) {
+    auto ins = [&](u32 col) {
         const u32 rc = hash_insert<BITS>(table, col);
         mine += rc == 1;
         full |= rc == 2;
-    });
+    };
+    for_each_product<WG, 8, BIG_ROW>(g, js, je, big, ins);
     __syncthreads();
     if (MODE == SYM) {
         if (full) atomicOr(o.overflow, 1u);
@@ -429,14 +468,13 @@ __global__ __launch_bounds__(WG) void k_hash(Mats g, const u32 *__restrict__ per
 // ---------------------------------------------------------------------------
 // BITMAP rows: an LDS bitmap of 2^WBITS words covers 32 * 2^WBITS columns;
 // wider column ranges are processed in slices.
+// One row (position idx of the bitmap bins) of the bitmap kernel.
 template <int WBITS, int WG, int MODE>
-__global__ __launch_bounds__(WG) void k_bitmap(Mats g, const u32 *__restrict__ perm, Out o) {
+__device__ __forceinline__ void bitmap_row(const Mats &g, const u32 *__restrict__ perm, u32 idx, const Out &o,
+                                           u32 *bm, u32 *big, u32 *red) {
     constexpr u32 W = 1u << WBITS;
     constexpr u64 SPAN = 32ull * W;
-    __shared__ __align__(16) u32 bm[W];
-    __shared__ __align__(8) u32 big[BIG_WORDS(WG)];
-    __shared__ u32 red[WG / 64 + 1];
-    const u32 r = perm[blockIdx.x];
+    const u32 r = perm[idx];
     const u32 i = g.row0 + r;
     const u32 js = g.a_rpt[i], je = g.a_rpt[i + 1];
     const u32 nslices = (u32)(((u64)g.ncols + SPAN - 1) / SPAN);
@@ -452,10 +490,10 @@ __global__ __launch_bounds__(WG) void k_bitmap(Mats g, const u32 *__restrict__ p
         }
         __syncthreads();
         if (nslices == 1)
-            for_each_product<WG, 16>(g, js, je, big,
-                                     [&](u32 col) { atomicOr(&bm[col >> 5], 1u << (col & 31)); });
+            for_each_product<WG, (MODE == SYM ? 16 : 8), LONG_ROW>(
+                g, js, je, big, [&](u32 col) { atomicOr(&bm[col >> 5], 1u << (col & 31)); });
         else
-            for_each_product<WG, 16>(g, js, je, big, [&](u32 col) {
+            for_each_product<WG, (MODE == SYM ? 16 : 8), LONG_ROW>(g, js, je, big, [&](u32 col) {
                 const u64 d = (u64)col - base;
                 if (d < SPAN) atomicOr(&bm[(u32)(d >> 5)], 1u << (col & 31));
             });
@@ -465,7 +503,7 @@ __global__ __launch_bounds__(WG) void k_bitmap(Mats g, const u32 *__restrict__ p
             constexpr u32 WPT = W / WG, TPC = EMIT_CHUNK / WPT, NCH = W / EMIT_CHUNK;
             const uint4 *bm4 = reinterpret_cast<const uint4 *>(bm);
             u32 cnt = 0;
-#pragma unroll
+#pragma unroll 2
             for (u32 q = 0; q < WPT / 4; ++q) {
                 const uint4 v = bm4[threadIdx.x * (WPT / 4) + q];
                 cnt += __popc(v.x) + __popc(v.y) + __popc(v.z) + __popc(v.w);
@@ -477,7 +515,7 @@ __global__ __launch_bounds__(WG) void k_bitmap(Mats g, const u32 *__restrict__ p
             // numeric pass: only a row whose count matches the symbolic pass
             // (its slot size) is dumped; anything else takes the capped path below
             if (n >= o.dump.dense_min && (MODE == ONE ? n <= lim : n == lim)) {
-                const u32 slot = blockIdx.x;
+                const u32 slot = idx;
                 uint4 *dst = o.dump.store + (u64)slot * (W / 4);
                 for (u32 q = threadIdx.x; q < W / 4; q += WG) dst[q] = bm4[q];
                 if (threadIdx.x % TPC == 0) o.dump.chunk_cnt[(u64)slot * NCH + threadIdx.x / TPC] = csum;
@@ -523,40 +561,12 @@ __global__ __launch_bounds__(WG) void k_bitmap(Mats g, const u32 *__restrict__ p
                 }
                 return;
             }
-            u64 pos = written + woff;
-            const u32 hi = (u32)l >> 5, bit = (u32)l & 31u;
-            const u32 below = (1u << bit) - 1u;
+            const u64 pos = written + woff;
             const u32 dense = g.emit ? g.emit : EMIT_DEFAULT;
+            u64 done = 0;
             for (u32 q0 = 0; q0 < WPW; q0 += dev::WAVE) {
                 const u32 word = wb[q0 + l];
-                const u32 pc = (u32)__popc(word);
-                const u32 inc = dev::wave_incl_scan(pc);
-                const u32 st = __shfl(inc, dev::WAVE - 1, dev::WAVE);
-                const u32 exc = inc - pc;
-                const u32 cb = (u32)(base + 32ull * (wv * WPW + q0));
-                if (st >= dense) {
-#pragma unroll 4
-                    for (u32 k = 0; k < 32; ++k) {
-                        const u32 wi = 2 * k + hi;
-                        const u32 v = __shfl(word, (int)wi, dev::WAVE);
-                        const u32 eo = __shfl(exc, (int)wi, dev::WAVE);
-                        if ((v >> bit) & 1u) {
-                            const u64 q = pos + eo + (u32)__popc(v & below);
-                            if (q < lim) out[q] = cb + 32u * wi + bit;
-                        }
-                    }
-                } else if (st) {
-                    u32 w = word;
-                    u64 q = pos + exc;
-                    const u32 c0 = cb + 32u * (u32)l;
-                    while (w) {
-                        const u32 b2 = __ffs(w) - 1;
-                        if (q < lim) out[q] = c0 + b2;
-                        ++q;
-                        w &= w - 1;
-                    }
-                }
-                pos += st;
+                done += dev::emit_step(word, (u32)(base + 32ull * (wv * WPW + q0)), pos + done, out, lim, dense);
             }
             written += tot;
         }
@@ -574,6 +584,17 @@ __global__ __launch_bounds__(WG) void k_bitmap(Mats g, const u32 *__restrict__ p
             atomicOr(o.overflow, 2u);
         }
     }
+}
+
+// BITMAP rows: one workgroup per row (a launch of one workgroup per row lets
+// the dispatcher balance rows of very different sizes over the CUs and
+// interleave the concurrent hash launches; a persistent grid did neither).
+template <int WBITS, int WG, int MODE>
+__global__ __launch_bounds__(WG) void k_bitmap(Mats g, const u32 *__restrict__ perm, Out o) {
+    __shared__ __align__(16) u32 bm[1u << WBITS];
+    __shared__ __align__(8) u32 big[BIG_WORDS(WG)];
+    __shared__ u32 red[WG / 64 + 1];
+    bitmap_row<WBITS, WG, MODE>(g, perm, blockIdx.x, o, bm, big, red);
 }
 
 }  // namespace spg
