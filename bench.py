@@ -99,13 +99,15 @@ def run_steps(cb, ctl, a_dev, blocks, mode, n, timing):
     for _ in range(n):
         agg = {"products": 0, "nnz": 0, "ms_kernel": [0.0] * 16, "launches": [0] * 16,
                "bin_rows": [0] * 32, "bin_products": [0] * 32, "bin_a_nnz": [0] * 32, "bin_out_nnz": [0] * 32,
-               "fallback_rows": 0}
+               "fallback_rows": 0, "dense_rows": 0, "dense_nnz": 0}
         for lo, hi in blocks:
             C, st = cb.dcsr.spgemm(ctl, a_dev, a_dev, mode=mode, row_begin=lo, row_end=hi, timing=timing)
             del C
             agg["products"] += st["products"]
             agg["nnz"] += st["nnz"]
             agg["fallback_rows"] += st["fallback_rows"]
+            agg["dense_rows"] += st["dense_rows"]
+            agg["dense_nnz"] += st["dense_nnz"]
             for k in ("ms_kernel", "launches", "bin_rows", "bin_products", "bin_a_nnz", "bin_out_nnz"):
                 agg[k] = [x + y for x, y in zip(agg[k], st[k])]
     ctl.sync()
@@ -335,13 +337,19 @@ def main():
     avg_ms = agg["ms_kernel"][fam] / launches
     # algorithmic bytes of that family's launches: A rows (rpt 8 B/row,
     # a_cols 4 B + B row extent 8 B per A nonzero), 4 B per product read from
-    # B, 4 B per output column written (not in the symbolic pass)
+    # B, 4 B per output column the family itself writes.  Dense bitmap rows
+    # are written by the emission family (4 B per column), not by the bitmap
+    # family that found them; the bitmap dump in between is not counted.
     pass_, kind = divmod(fam, 4)
     bins = {0: range(1, 2), 1: range(2, 10), 2: range(10, 32)}.get(kind, range(0))
     writes = pass_ != 0
     rows_f = sum(agg["bin_rows"][b] for b in bins)
     bytes_total = sum(12 * agg["bin_a_nnz"][b] + 4 * agg["bin_products"][b] +
                       (4 * agg["bin_out_nnz"][b] if writes else 0) for b in bins) + 8 * rows_f
+    if kind == 2 and writes:
+        bytes_total -= 4 * agg["dense_nnz"]
+    if kind == 3:
+        bytes_total = 4 * agg["dense_nnz"]
     bytes_per_launch = bytes_total / launches
     achieved = bytes_per_launch / (avg_ms / 1e3) / 1e9
     traffic = None
@@ -367,7 +375,8 @@ def main():
         "output_nnz_per_s": R["nnz_per_s"],
         "modes": {m: {"ms_per_step": v["ms_per_step"], "products_per_s": v["products_per_s"],
                       "ms_kernel": dict(zip(fam_names, v["agg"]["ms_kernel"])),
-                      "fallback_rows": v["agg"]["fallback_rows"], "tuning": tuned[m]} for m, v in results.items()},
+                      "fallback_rows": v["agg"]["fallback_rows"], "dense_rows": v["agg"]["dense_rows"],
+                      "dense_nnz": v["agg"]["dense_nnz"], "tuning": tuned[m]} for m, v in results.items()},
         "predictor_training": train,
         "validated_rows": validated,
         "roofline": roofline,

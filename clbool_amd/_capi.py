@@ -55,7 +55,8 @@ class SpgemmStats(C.Structure):
                 ("pred_abs_log2_err", C.c_double),
                 ("ms_setup", C.c_double), ("ms_symbolic", C.c_double), ("ms_numeric", C.c_double),
                 ("ms_assemble", C.c_double), ("ms_total", C.c_double),
-                ("ms_kernel", C.c_double * 16), ("launches", C.c_uint32 * 16), ("capacity", C.c_uint64)]
+                ("ms_kernel", C.c_double * 16), ("launches", C.c_uint32 * 16), ("capacity", C.c_uint64),
+                ("dense_rows", C.c_uint64), ("dense_nnz", C.c_uint64)]
 
     _ARRAYS = ("bin_rows", "bin_products", "bin_a_nnz", "bin_out_nnz", "ms_kernel", "launches")
 

@@ -164,8 +164,23 @@ __device__ __forceinline__ uint32_t emit_step(uint32_t word, uint32_t cb, uint64
     const uint32_t inc = wave_incl_sum_dpp(pc);
     const uint32_t st = (uint32_t)__builtin_amdgcn_readlane((int)inc, 63);
     if (st == 0) return 0;
+    if (pos + st > lim) {
+        // only if the row's slot is too small (the passes disagreed): every
+        // store is checked
+        uint32_t w = word;
+        uint64_t q = pos + (inc - pc);
+        const uint32_t c0 = cb + 32u * l;
+        while (w) {
+            if (q < lim) out[q] = c0 + (uint32_t)__builtin_ctz(w);
+            ++q;
+            w &= w - 1u;
+        }
+        return st;
+    }
+    // the step fits: stores relative to one uniform base, no checks
+    uint32_t *o = out + pos;
     if (st >= dense) {
-        uint64_t p = pos;
+        uint32_t p = 0;
         const uint32_t sh = l & 31u;
         for (uint32_t k = 0; k < 32; ++k) {
             const uint32_t w0 = (uint32_t)__builtin_amdgcn_readlane((int)word, (int)(2 * k));
@@ -174,18 +189,16 @@ __device__ __forceinline__ uint32_t emit_step(uint32_t word, uint32_t cb, uint64
             const uint32_t v = l < 32 ? w0 : w1;
             const bool set = (v >> sh) & 1u;
             const uint64_t m = __ballot(set);
-            const uint64_t q = p + lanes_below(m);
-            if (set && q < lim) out[q] = cb + 64u * k + l;
-            p += (uint64_t)__popcll(m);
+            if (set) o[p + lanes_below(m)] = cb + 64u * k + l;
+            p += (uint32_t)__popcll(m);
         }
     } else {
         uint32_t w = word;
-        uint64_t q = pos + (inc - pc);
+        uint32_t boff = (inc - pc) * 4u;
         const uint32_t c0 = cb + 32u * l;
         while (w) {
-            const uint32_t b = (uint32_t)__ffs(w) - 1u;
-            if (q < lim) out[q] = c0 + b;
-            ++q;
+            *reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(o) + boff) = c0 + (uint32_t)__builtin_ctz(w);
+            boff += 4u;
             w &= w - 1u;
         }
     }

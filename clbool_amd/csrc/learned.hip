@@ -462,7 +462,7 @@ void spgemm_learned(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &b, u
                                                        pred.ptr, u32p(cols), err.ptr);
         CLB_LAUNCHED("compact_rows");
         lt.end((int)CLB_KF(CLB_KP_FALLBACK, CLB_KK_EMIT), ctl->stream);
-        emit_dense(ctl, perm.ptr, h, dump, offs.ptr, u32p(cols), &lt, CLB_KP_ONEPASS);
+        emit_dense(ctl, perm.ptr, h, dump, offs.ptr, u32p(cols), &lt, CLB_KP_ONEPASS, timing ? &st : nullptr);
         assemble(ctl, s, a, nnz_row.ptr, offs.ptr, pos.ptr, nnz, nzr, cols, out);
         double e = 0;
         readback(ctl, &e, err.ptr, 8);

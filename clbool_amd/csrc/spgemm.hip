@@ -130,42 +130,35 @@ __global__ void k_set_positions(const u32 *__restrict__ nnz_row, const u64 *__re
     c_rows[q] = a_rows[row0 + i];
 }
 
-// Expands one 1024-word chunk of a dumped bitmap row into its columns at
-// their final positions: each of the 4 waves owns 256 words, places them by
-// the waves' popcounts, and writes them 64 words at a time (dev::emit_step).
+// Expands one dumped bitmap row (one workgroup per row) into its columns at
+// their final positions.  Each of the 16 waves owns a contiguous sixteenth of
+// the row's words; its start position is the prefix of the chunk counts
+// before its first chunk, so the waves need no exchange.  A wave loads its
+// words 8 steps at a time and writes them with dev::emit_step.
 // gpre = exclusive prefix of chunk_cnt over all slots.
-constexpr int EMIT_WG = 256;
+constexpr int EMIT_WG = 1024;
 __global__ __launch_bounds__(EMIT_WG) void k_bitmap_emit(const uint4 *__restrict__ store, u32 nch,
-                                                         const u32 *__restrict__ chunk_cnt,
                                                          const u64 *__restrict__ gpre, const u32 *__restrict__ list,
                                                          const u32 *__restrict__ perm, const u64 *__restrict__ off,
                                                          u32 words_per_row, u32 dense, u32 *__restrict__ cols) {
-    constexpr u32 NW = EMIT_WG / 64, WPW = EMIT_CHUNK / NW, STEPS = WPW / 64;
-    __shared__ u32 red[NW];
-    const u32 li = blockIdx.x / nch, c = blockIdx.x % nch;
-    const u32 slot = list[li];
-    const u64 ci = (u64)slot * nch + c;
-    if (chunk_cnt[ci] == 0) return;
+    constexpr u32 NW = EMIT_WG / 64, BATCH = 8;
+    const u32 slot = list[blockIdx.x];
     const u32 r = perm[slot];
     u32 *out = cols + off[r];
     const u64 lim = off[r + 1] - off[r];
     const u32 wv = dev::wave_id(), l = (u32)dev::lane();
-    const u32 *words = reinterpret_cast<const u32 *>(store) + (u64)slot * words_per_row + (u64)c * EMIT_CHUNK + wv * WPW;
-    u32 w[STEPS];
-    u32 cnt = 0;
+    const u32 wpw = words_per_row / NW;  // a multiple of EMIT_CHUNK: the window is >= 2^14 words
+    const u32 first = wv * wpw;
+    const u32 *words = reinterpret_cast<const u32 *>(store) + (u64)slot * words_per_row + first;
+    u64 p = gpre[(u64)slot * nch + first / EMIT_CHUNK] - gpre[(u64)slot * nch];
+    for (u32 s0 = 0; s0 < wpw; s0 += BATCH * 64) {
+        u32 w[BATCH];
 #pragma unroll
-    for (u32 k = 0; k < STEPS; ++k) {
-        w[k] = words[k * 64 + l];
-        cnt += __popc(w[k]);
+        for (u32 k = 0; k < BATCH; ++k) w[k] = words[s0 + k * 64 + l];
+#pragma unroll
+        for (u32 k = 0; k < BATCH; ++k)
+            p += dev::emit_step(w[k], (first + s0 + k * 64) * 32u, p, out, lim, dense);
     }
-    cnt = dev::wave_sum(cnt);
-    if (l == 0) red[wv] = cnt;
-    __syncthreads();
-    u64 p = gpre[ci] - gpre[(u64)slot * nch];
-    for (u32 v = 0; v < wv; ++v) p += red[v];
-    const u32 colbase = (c * EMIT_CHUNK + wv * WPW) * 32u;
-#pragma unroll
-    for (u32 k = 0; k < STEPS; ++k) p += dev::emit_step(w[k], colbase + k * 64u * 32u, p, out, lim, dense);
 }
 
 // ---------------------------------------------------------------------------
@@ -199,20 +192,25 @@ void setup_dump(clb_controls_s *ctl, const Setup &s, const BinHost &h, DumpBufs 
 }
 
 void emit_dense(clb_controls_s *ctl, const u32 *perm, const BinHost &h, DumpBufs &d, const u64 *off, u32 *cols,
-                LaunchTimer *lt, int pass) {
+                LaunchTimer *lt, int pass, clb_spgemm_stats *st) {
     if (!d.on) return;
     u32 count = 0;
     readback(ctl, &count, d.count.ptr, 4);
     if (!count) return;
     DevArray<u64> gpre(ctl, d.cnt.n + 1);
-    exclusive_scan_u32_to_u64(ctl, d.cnt.ptr, gpre.ptr, d.cnt.n, nullptr);
-    if (lt) lt->begin(ctl->stream);
+    exclusive_scan_u32_to_u64(ctl, d.cnt.ptr, gpre.ptr, d.cnt.n, gpre.ptr + d.cnt.n);
     const u32 dense = ctl->tuning.emit_lane_bits_min ? ctl->tuning.emit_lane_bits_min : EMIT_DEFAULT;
-    k_bitmap_emit<<<(u64)count * d.nch, EMIT_WG, 0, ctl->stream>>>(d.store.ptr, d.nch, d.cnt.ptr, gpre.ptr,
-                                                                    d.list.ptr, perm + h.off[BIN_BITMAP0], off,
-                                                                    d.words, dense, cols);
+    if (lt) lt->begin(ctl->stream);
+    k_bitmap_emit<<<count, EMIT_WG, 0, ctl->stream>>>(d.store.ptr, d.nch, gpre.ptr, d.list.ptr,
+                                                       perm + h.off[BIN_BITMAP0], off, d.words, dense, cols);
     CLB_LAUNCHED("bitmap_emit");
     if (lt) lt->end((int)CLB_KF(pass, CLB_KK_EMIT), ctl->stream);
+    if (st) {
+        u64 total = 0;
+        readback(ctl, &total, gpre.ptr + d.cnt.n, 8);
+        st->dense_rows += count;
+        st->dense_nnz += total;
+    }
 }
 
 BinCfg make_bincfg(const clb_controls_s *ctl, u32 ncols) {
@@ -574,7 +572,7 @@ void spgemm_plain(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &b, u32
         DumpBufs dump;
         setup_dump(ctl, s, h2, dump, o2);
         launch_bins(ctl, s, perm2.ptr, h2.off, NUM, o2, &lt, -1);
-        emit_dense(ctl, perm2.ptr, h2, dump, offs.ptr, u32p(cols), &lt, CLB_KP_NUMERIC);
+        emit_dense(ctl, perm2.ptr, h2, dump, offs.ptr, u32p(cols), &lt, CLB_KP_NUMERIC, timing ? &st : nullptr);
         tm.mark();  // 4
         assemble(ctl, s, a, nnz_row.ptr, offs.ptr, pos.ptr, nnz, nzr, cols, out);
         tm.mark();  // 5

@@ -58,7 +58,7 @@ struct DumpBufs {
 void setup_dump(clb_controls_s *ctl, const Setup &s, const BinHost &h, DumpBufs &d, Out &o);
 // Expands the dumped rows into cols at the exact row offsets `off`.
 void emit_dense(clb_controls_s *ctl, const u32 *perm, const BinHost &h, DumpBufs &d, const u64 *off, u32 *cols,
-                LaunchTimer *lt, int pass);
+                LaunchTimer *lt, int pass, clb_spgemm_stats *st);
 
 BinCfg make_bincfg(const clb_controls_s *ctl, u32 ncols);
 // B row map, kernel arguments and (count == true) per-row product counts.

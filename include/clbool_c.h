@@ -109,6 +109,8 @@ typedef struct clb_spgemm_stats {
     double   ms_kernel[16];
     uint32_t launches[16];
     uint64_t capacity;      /* learned mode: columns reserved by the predicted sizes */
+    uint64_t dense_rows;    /* bitmap rows written by the dense-row emission kernel */
+    uint64_t dense_nnz;     /* their output nonzeros                         */
 } clb_spgemm_stats;
 
 /* kernel families of clb_spgemm_stats.ms_kernel: pass x kind */
