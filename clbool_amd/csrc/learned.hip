@@ -50,57 +50,23 @@ static Learned *state(clb_controls_s *ctl) {
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ float flog(u32 x) { return log2f(1.0f + (float)x) * FSCALE; }
 
-// Per-row products and the five features; G lanes share a row.
-template <int G>
-__global__ __launch_bounds__(256) void k_features(Mats g, u32 nrows, u32 *__restrict__ prod, u64 *__restrict__ total,
-                                                  float *__restrict__ feat) {
-    const u32 gid = blockIdx.x * 256 + threadIdx.x;
-    const u32 row = gid / G, sub = gid % G;
-    u32 p = 0, mn = 0xFFFFFFFFu, mx = 0, fc = 0xFFFFFFFFu, lc = 0;
-    u32 alen = 0;
-    if (row < nrows) {
-        const u32 i = g.row0 + row;
-        const u32 js = g.a_rpt[i], je = g.a_rpt[i + 1];
-        alen = je - js;
-        for (u32 j = js + sub; j < je; j += G) {
-            const u32 k = g.a_cols[j];
-            if (k < g.mapn) {
-                const uint2 r = g.brange[k];
-                const u32 len = r.y - r.x;
-                if (len) {
-                    p += len;
-                    mn = len < mn ? len : mn;
-                    mx = len > mx ? len : mx;
-                    const u32 c0 = g.b_cols[r.x], c1 = g.b_cols[r.y - 1];
-                    fc = c0 < fc ? c0 : fc;
-                    lc = c1 > lc ? c1 : lc;
-                }
-            }
-        }
-    }
-#pragma unroll
-    for (int o = G / 2; o > 0; o >>= 1) {
-        p += __shfl_xor(p, o, G);
-        const u32 a = __shfl_xor(mn, o, G), b = __shfl_xor(mx, o, G);
-        const u32 c = __shfl_xor(fc, o, G), d = __shfl_xor(lc, o, G);
-        mn = a < mn ? a : mn;
-        mx = b > mx ? b : mx;
-        fc = c < fc ? c : fc;
-        lc = d > lc ? d : lc;
-    }
-    const bool lead = row < nrows && sub == 0;
-    if (lead) {
-        prod[row] = p;
-        const u32 span = p ? lc - fc + 1 : 0;
-        float *f = feat + (u64)row * NF;
-        f[0] = flog(p);
-        f[1] = flog(alen);
-        f[2] = flog(p ? mn : 0);
-        f[3] = flog(mx);
-        f[4] = flog(span);
-    }
-    u64 s = dev::wave_sum<u64>(lead ? (u64)p : 0ull);
-    if (dev::lane() == 0 && s) atomicAdd(reinterpret_cast<unsigned long long *>(total), (unsigned long long)s);
+// Feature vectors from the per-row statistics of k_row_stats: log2(1 + x)/32
+// of products, A-row length, min and max B-row length, and column span (last
+// column of any of the row's B rows minus the first, plus one).
+__global__ void k_features(Mats g, u32 nrows, const u32 *__restrict__ prod, const u32 *__restrict__ mn,
+                           const u32 *__restrict__ mx, const u32 *__restrict__ fc, const u32 *__restrict__ lc,
+                           float *__restrict__ feat) {
+    const u32 row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= nrows) return;
+    const u32 i = g.row0 + row;
+    const u32 p = prod[row];
+    const u32 span = p ? lc[row] - fc[row] + 1 : 0;
+    float *f = feat + (u64)row * NF;
+    f[0] = flog(p);
+    f[1] = flog(g.a_rpt[i + 1] - g.a_rpt[i]);
+    f[2] = flog(p ? mn[row] : 0);
+    f[3] = flog(mx[row]);
+    f[4] = flog(span);
 }
 
 // MLP forward of one sample with parameters in LDS.
@@ -330,16 +296,17 @@ static void prepare_features(clb_controls_s *ctl, const clb_dcsr &a, const clb_d
     // feature kernel (it also writes s.prod and the product total)
     prepare(ctl, a, b, rb, nrows, s, false);
     feat.reset(ctl, (u64)nrows * NF);
-    const double mean = a.nzr ? (double)a.nnz / a.nzr : 1.0;
-    int G = 1;
-    while (G < 64 && G * 2 <= mean) G *= 2;
-    const u32 grid = div_up((u64)nrows * G, 256);
-    switch (G) {
-#define FK(GG) \
-    case GG: k_features<GG><<<grid, 256, 0, ctl->stream>>>(s.g, nrows, s.prod.ptr, s.scal.ptr, feat.ptr); break;
-        FK(1) FK(2) FK(4) FK(8) FK(16) FK(32) FK(64)
-#undef FK
-    }
+    DevArray<u32> mn(ctl, nrows), mx(ctl, nrows), fc(ctl, nrows), lc(ctl, nrows);
+    fill_u32(ctl, s.prod.ptr, nrows, 0u);
+    fill_u32(ctl, mn.ptr, nrows, 0xFFFFFFFFu);
+    fill_u32(ctl, mx.ptr, nrows, 0u);
+    fill_u32(ctl, fc.ptr, nrows, 0xFFFFFFFFu);
+    fill_u32(ctl, lc.ptr, nrows, 0u);
+    k_row_stats<true><<<div_up(a.nnz, RS_WG), RS_WG, 0, ctl->stream>>>(s.g, nrows, s.prod.ptr, s.scal.ptr, mn.ptr,
+                                                                       mx.ptr, fc.ptr, lc.ptr);
+    CLB_LAUNCHED("row_stats");
+    k_features<<<div_up(nrows, 256), 256, 0, ctl->stream>>>(s.g, nrows, s.prod.ptr, mn.ptr, mx.ptr, fc.ptr, lc.ptr,
+                                                             feat.ptr);
     CLB_LAUNCHED("features");
 }
 

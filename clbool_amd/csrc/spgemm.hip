@@ -24,32 +24,6 @@ __global__ void k_brange_set(const u32 *__restrict__ b_rpt, const u32 *__restric
     if (r < mapn) brange[r] = make_uint2(b_rpt[i], b_rpt[i + 1]);
 }
 
-// count_workload (count_workload.cl:30-53): products of each A row; G lanes
-// share a row.
-template <int G>
-__global__ __launch_bounds__(256) void k_count(Mats g, u32 nrows, u32 *__restrict__ prod, u64 *__restrict__ total) {
-    const u32 gid = blockIdx.x * 256 + threadIdx.x;
-    const u32 row = gid / G, sub = gid % G;
-    u32 p = 0;
-    if (row < nrows) {
-        const u32 i = g.row0 + row;
-        const u32 je = g.a_rpt[i + 1];
-        for (u32 j = g.a_rpt[i] + sub; j < je; j += G) {
-            const u32 k = g.a_cols[j];
-            if (k < g.mapn) {
-                const uint2 r = g.brange[k];
-                p += r.y - r.x;
-            }
-        }
-    }
-#pragma unroll
-    for (int o = G / 2; o > 0; o >>= 1) p += __shfl_xor(p, o, G);
-    const bool lead = row < nrows && sub == 0;
-    if (lead) prod[row] = p;
-    u64 s = dev::wave_sum<u64>(lead ? (u64)p : 0ull);
-    if (dev::lane() == 0 && s) atomicAdd(reinterpret_cast<unsigned long long *>(total), (unsigned long long)s);
-}
-
 // Histogram of bins.  key == nullptr: key = min(P, ncols) (symbolic sizing).
 // Rows in BIN_EMPTY get nnz_row = 0 when zero_empty is set.
 __global__ __launch_bounds__(256) void k_bin_hist(Mats g, const u32 *__restrict__ P, const u32 *__restrict__ key,
@@ -255,17 +229,11 @@ void prepare(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &b, u32 row_
     s.scal.reset(ctl, 8);  // [0] total products, [1] nnz total, [2] nzr total, [3] overflow flag
     fill_u64(ctl, s.scal.ptr, 8, 0ull);
     if (!count) return;
-    // lanes per row from the mean row length of the matrix
-    const double mean = a.nzr ? (double)a.nnz / a.nzr : 1.0;
-    int G = 1;
-    while (G < 64 && G * 2 <= mean) G *= 2;
-    const u32 grid = div_up((u64)nrows * G, 256);
-    switch (G) {
-#define CNT(GG) \
-    case GG: k_count<GG><<<grid, 256, 0, ctl->stream>>>(s.g, nrows, s.prod.ptr, s.scal.ptr); break;
-        CNT(1) CNT(2) CNT(4) CNT(8) CNT(16) CNT(32) CNT(64)
-#undef CNT
-    }
+    // count_workload (count_workload.cl:30-53): products of each A row,
+    // balanced over A's nonzeros (k_row_stats)
+    fill_u32(ctl, s.prod.ptr, nrows, 0u);
+    k_row_stats<false><<<div_up(a.nnz, RS_WG), RS_WG, 0, ctl->stream>>>(s.g, nrows, s.prod.ptr, s.scal.ptr, nullptr,
+                                                                        nullptr, nullptr, nullptr);
     CLB_LAUNCHED("count_workload");
 }
 

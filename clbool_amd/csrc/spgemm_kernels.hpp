@@ -257,6 +257,99 @@ __device__ __forceinline__ void for_each_product(const Mats &g, u32 js, u32 je, 
     }
 }
 
+// ---------------------------------------------------------------------------
+// Per-row statistics of rows [row0, row0 + nrows) of A over its B rows,
+// load-balanced by nonzeros: workgroup b takes A nonzeros [j0, j0 + 256) of
+// the rows' range, finds their rows from a copy of a_rpt in LDS, reduces per
+// row in LDS and merges into the per-row arrays with one global atomic per
+// row and workgroup (a row spread over several workgroups is merged from
+// each).  A hub row of A with many nonzeros thus costs no more than as many
+// short rows, where one thread group per row would walk it serially.
+//   prod[r] += products (must start at 0); total += all products;
+//   with FEAT also mn/mx (min/max B-row length, start at ~0 / 0) and fc/lc
+//   (first column of any of its B rows / last column, start at ~0 / 0).
+constexpr u32 RS_WG = 256;
+template <bool FEAT>
+__global__ __launch_bounds__(RS_WG) void k_row_stats(Mats g, u32 nrows, u32 *__restrict__ prod,
+                                                     u64 *__restrict__ total, u32 *__restrict__ mn,
+                                                     u32 *__restrict__ mx, u32 *__restrict__ fc,
+                                                     u32 *__restrict__ lc) {
+    __shared__ u32 rs[RS_WG + 2];
+    __shared__ u32 sp[RS_WG + 1], smn[RS_WG + 1], smx[RS_WG + 1], sfc[RS_WG + 1], slc[RS_WG + 1];
+    __shared__ u32 r0s;
+    __shared__ u64 red[RS_WG / 64 + 1];
+    const u32 rbeg = g.row0, rend = g.row0 + nrows;
+    const u32 jbase = g.a_rpt[rbeg], jend = g.a_rpt[rend];
+    const u64 j0 = (u64)jbase + (u64)blockIdx.x * RS_WG;
+    if (j0 >= jend) return;  // uniform: the grid covers all of A's nonzeros
+    if (threadIdx.x == 0) {
+        // last row r in [rbeg, rend) with a_rpt[r] <= j0
+        u32 lo = rbeg, hi = rend;
+        while (hi - lo > 1) {
+            const u32 mid = lo + (hi - lo) / 2;
+            if (g.a_rpt[mid] <= j0) lo = mid;
+            else hi = mid;
+        }
+        r0s = lo;
+    }
+    __syncthreads();
+    const u32 r0 = r0s;
+    // rows r0 .. r0 + nr - 1 can hold this workgroup's nonzeros (every stored
+    // row has one, so at most RS_WG + 1 of them)
+    const u32 nr = rend - r0 < RS_WG + 1 ? rend - r0 : RS_WG + 1;
+    for (u32 t = threadIdx.x; t <= nr; t += RS_WG) rs[t] = g.a_rpt[r0 + t];
+    for (u32 t = threadIdx.x; t < nr; t += RS_WG) {
+        sp[t] = 0;
+        if (FEAT) {
+            smn[t] = 0xFFFFFFFFu;
+            smx[t] = 0;
+            sfc[t] = 0xFFFFFFFFu;
+            slc[t] = 0;
+        }
+    }
+    __syncthreads();
+    const u64 j = j0 + threadIdx.x;
+    u32 mine = 0;
+    if (j < jend) {
+        // local row: last t with rs[t] <= j
+        u32 lo = 0, hi = nr;
+        while (hi - lo > 1) {
+            const u32 mid = (lo + hi) / 2;
+            if (rs[mid] <= j) lo = mid;
+            else hi = mid;
+        }
+        const u32 k = g.a_cols[j];
+        if (k < g.mapn) {
+            const uint2 b = g.brange[k];
+            const u32 len = b.y - b.x;
+            if (len) {
+                mine = len;
+                atomicAdd(&sp[lo], len);
+                if (FEAT) {
+                    atomicMin(&smn[lo], len);
+                    atomicMax(&smx[lo], len);
+                    atomicMin(&sfc[lo], g.b_cols[b.x]);
+                    atomicMax(&slc[lo], g.b_cols[b.y - 1]);
+                }
+            }
+        }
+    }
+    const u64 bs = dev::block_sum<u64, RS_WG>((u64)mine, red);
+    if (threadIdx.x == 0 && bs) atomicAdd(reinterpret_cast<unsigned long long *>(total), (unsigned long long)bs);
+    for (u32 t = threadIdx.x; t < nr; t += RS_WG) {
+        // only rows that overlap this workgroup's nonzeros
+        if (rs[t + 1] <= j0 || rs[t] >= j0 + RS_WG || rs[t] >= jend) continue;
+        const u32 row = r0 + t - rbeg;
+        if (sp[t]) atomicAdd(&prod[row], sp[t]);
+        if (FEAT && sp[t]) {
+            atomicMin(&mn[row], smn[t]);
+            atomicMax(&mx[row], smx[t]);
+            atomicMin(&fc[row], sfc[t]);
+            atomicMax(&lc[row], slc[t]);
+        }
+    }
+}
+
 // Insert into an LDS open-addressing table of 2^BITS slots (linear probing,
 // EMPTY marks a free slot).  Returns 1 if col was new, 0 if present, and 2
 // if the table is full (only possible when a capacity guess was too small).
