@@ -299,7 +299,7 @@ def main():
             log(f"[bench] {m}: tuning from {args.tuning_file}: {tuned[m]}")
         elif not args.no_autotune:
             sample = blocks[::max(1, len(blocks) // 3)][:4]
-            res = at.autotune(ctl, a_dev, a_dev, sample, mode=code[m], repeats=1, log=log)
+            res = at.autotune(ctl, a_dev, a_dev, sample, mode=code[m], repeats=2, log=log)
             tuned[m] = res["tuning"]
             log(f"[bench] {m}: autotuned {res['tuning']} over {res['evaluated']} settings")
         else:

@@ -39,7 +39,8 @@ class Tuning(C.Structure):
     _fields_ = [("hash_max_slots", C.c_uint32), ("bitmap_min_products", C.c_uint32),
                 ("bitmap_slot_ratio", C.c_uint32), ("tiny_max_products", C.c_uint32),
                 ("learned_margin_pct", C.c_uint32), ("emit_lane_bits_min", C.c_uint32),
-                ("dump_bits_per_chunk", C.c_uint32), ("reserved", C.c_uint32 * 9)]
+                ("dump_bits_per_chunk", C.c_uint32), ("bitmap_persistent", C.c_uint32),
+                ("reserved", C.c_uint32 * 8)]
 
 
 class SpgemmOpts(C.Structure):

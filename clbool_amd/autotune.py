@@ -19,6 +19,7 @@ import numpy as np
 KNOBS = {
     "emit_lane_bits_min": [256, 1024, 4096],   # 4096: never one lane per bit
     "dump_bits_per_chunk": [64, 256, 1024, 4096, 65536],   # 65536: no dump
+    "bitmap_persistent": [0, 1],
     "bitmap_min_products": [4096, 8192, 16384, 32768, 65536],
     "hash_max_slots": [4096, 8192, 16384],
     "bitmap_slot_ratio": [1, 2, 4],
@@ -59,8 +60,9 @@ def _checksum(cb, controls, c) -> tuple:
     return (int(c.nnz), int(head.astype(np.uint64).sum() ^ (tail.astype(np.uint64).sum() << np.uint64(1))))
 
 
-def autotune(controls, a, b, blocks, mode=0, repeats=2, knobs=None, log=None) -> dict:
-    """Coordinate descent over KNOBS on the given row blocks of A * B."""
+def autotune(controls, a, b, blocks, mode=0, repeats=2, knobs=None, log=None, passes=2) -> dict:
+    """Coordinate descent over KNOBS on the given row blocks of A * B (the
+    knobs interact, so the sweep over all of them runs `passes` times)."""
     import clbool_amd as cb
 
     knobs = knobs or [k for k in KNOBS if mode == cb.SPGEMM_LEARNED or k != "learned_margin_pct"]
@@ -83,7 +85,7 @@ def autotune(controls, a, b, blocks, mode=0, repeats=2, knobs=None, log=None) ->
     run()  # warm the caches and the allocator
     t_cur, ref = run()
     history = [(dict(cur), t_cur)]
-    for k in knobs:
+    for k in [k for _ in range(passes) for k in knobs]:
         for v in KNOBS[k]:
             if v == cur[k]:
                 continue

@@ -156,6 +156,7 @@ constexpr u32 EMPTY = 0xFFFFFFFFu;
 
 // Fill kernels (used instead of hipMemsetAsync so paths stay graph-safe).
 void fill_u32(clb_controls_s *ctl, u32 *p, u64 n, u32 v);
+void fill_u32_on(hipStream_t st, u32 *p, u64 n, u32 v);  // on a given stream
 void fill_u64(clb_controls_s *ctl, u64 *p, u64 n, u64 v);
 
 // Device-wide exclusive scans (scan.hip).  out may alias in.  If total is

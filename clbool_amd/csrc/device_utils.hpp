@@ -205,6 +205,40 @@ __device__ __forceinline__ uint32_t emit_step(uint32_t word, uint32_t cb, uint64
     return st;
 }
 
+// emit_step with the wave's columns staged in LDS: each lane writes its
+// word's columns into the wave's staging area `sg` (STAGE_WORDS words), then
+// the wave copies them out in order, 64 consecutive columns per store
+// instruction.  Scattered per-lane stores would each touch their own cache
+// line; the staged copy writes whole lines.  Column t of the step sits at
+// sg[t + t / 32]: lanes write about 16 apart in a dense step, which without
+// the padding word per 32 would put all 64 lanes on two LDS banks.  Returns
+// the step's set bits.
+constexpr uint32_t STAGE_WORDS = 2048 + 64;
+__device__ __forceinline__ uint32_t emit_step_staged(uint32_t word, uint32_t cb, uint64_t pos, uint32_t *out,
+                                                     uint64_t lim, uint32_t *sg) {
+    const uint32_t l = (uint32_t)lane();
+    const uint32_t pc = (uint32_t)__popc(word);
+    const uint32_t inc = wave_incl_sum_dpp(pc);
+    const uint32_t st = (uint32_t)__builtin_amdgcn_readlane((int)inc, 63);
+    if (st == 0) return 0;
+    uint32_t w = word, o = inc - pc;
+    const uint32_t c0 = cb + 32u * l;
+    while (w) {
+        sg[o + (o >> 5)] = c0 + (uint32_t)__builtin_ctz(w);
+        ++o;
+        w &= w - 1u;
+    }
+    __builtin_amdgcn_wave_barrier();
+    uint32_t *dst = out + pos;
+    // a row holds fewer than 2^32 columns
+    const uint32_t room = (uint32_t)(lim > pos ? lim - pos : 0);
+    const uint32_t n = st < room ? st : room;
+#pragma unroll 4
+    for (uint32_t t = l; t < n; t += (uint32_t)WAVE) dst[t] = sg[t + (t >> 5)];
+    __builtin_amdgcn_wave_barrier();
+    return st;
+}
+
 // Ascending bitonic sort of one value per lane across a wave (64 values).
 __device__ __forceinline__ uint32_t wave_sort64(uint32_t v) {
     const int l = lane();

@@ -184,6 +184,12 @@ void fill_u32(clb_controls_s *ctl, u32 *p, u64 n, u32 v) {
     CLB_LAUNCHED("fill_with");
 }
 
+void fill_u32_on(hipStream_t st, u32 *p, u64 n, u32 v) {
+    if (!n) return;
+    k_fill_u32<<<(u32)((n + 255) / 256 < 1024 ? (n + 255) / 256 : 1024), 256, 0, st>>>(p, n, v);
+    CLB_LAUNCHED("fill_with");
+}
+
 void fill_u64(clb_controls_s *ctl, u64 *p, u64 n, u64 v) {
     if (!n) return;
     k_fill_u64<<<fill_grid(ctl, n), 256, 0, ctl->stream>>>(p, n, v);

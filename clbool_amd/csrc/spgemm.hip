@@ -108,7 +108,9 @@ __global__ void k_set_positions(const u32 *__restrict__ nnz_row, const u64 *__re
 // their final positions.  Each of the 16 waves owns a contiguous sixteenth of
 // the row's words; its start position is the prefix of the chunk counts
 // before its first chunk, so the waves need no exchange.  A wave loads its
-// words 8 steps at a time and writes them with dev::emit_step.
+// words 8 steps at a time and writes them through its own LDS staging area
+// (dev::emit_step_staged), so every store instruction is one contiguous run
+// of columns.
 // gpre = exclusive prefix of chunk_cnt over all slots.
 constexpr int EMIT_WG = 1024;
 __global__ __launch_bounds__(EMIT_WG) void k_bitmap_emit(const uint4 *__restrict__ store, u32 nch,
@@ -116,6 +118,7 @@ __global__ __launch_bounds__(EMIT_WG) void k_bitmap_emit(const uint4 *__restrict
                                                          const u32 *__restrict__ perm, const u64 *__restrict__ off,
                                                          u32 words_per_row, u32 dense, u32 *__restrict__ cols) {
     constexpr u32 NW = EMIT_WG / 64, BATCH = 8;
+    __shared__ u32 stage[NW][dev::STAGE_WORDS];
     const u32 slot = list[blockIdx.x];
     const u32 r = perm[slot];
     u32 *out = cols + off[r];
@@ -131,7 +134,7 @@ __global__ __launch_bounds__(EMIT_WG) void k_bitmap_emit(const uint4 *__restrict
         for (u32 k = 0; k < BATCH; ++k) w[k] = words[s0 + k * 64 + l];
 #pragma unroll
         for (u32 k = 0; k < BATCH; ++k)
-            p += dev::emit_step(w[k], (first + s0 + k * 64) * 32u, p, out, lim, dense);
+            p += dev::emit_step_staged(w[k], (first + s0 + k * 64) * 32u, p, out, lim, stage[wv]);
     }
 }
 
@@ -226,6 +229,7 @@ void prepare(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &b, u32 row_
     s.g.emit = ctl->tuning.emit_lane_bits_min;
     s.cfg = make_bincfg(ctl, b.ncols);
     s.prod.reset(ctl, nrows);
+    s.ctr.reset(ctl, 4);
     s.scal.reset(ctl, 8);  // [0] total products, [1] nnz total, [2] nzr total, [3] overflow flag
     fill_u64(ctl, s.scal.ptr, 8, 0ull);
     if (!count) return;
@@ -338,9 +342,26 @@ LaunchTimer::~LaunchTimer() {
 
 // Launch the per-bin kernels of one pass.  off[b]..off[b+1] is bin b's range
 // of perm.  numeric == false: write nnz_row; true: write columns.
+// One workgroup per row, or (tuning.bitmap_persistent) a persistent grid
+// that takes rows from a counter.
 template <int WBITS, int WG, int MODE>
-static void launch_bitmap(hipStream_t st, const Mats &g, const u32 *perm, u32 n, const Out &o) {
-    k_bitmap<WBITS, WG, MODE><<<n, WG, 0, st>>>(g, perm, o);
+static void launch_bitmap(clb_controls_s *ctl, hipStream_t st, const Setup &s, const u32 *perm, u32 n,
+                          const Out &o) {
+    if (!ctl->tuning.bitmap_persistent) {
+        k_bitmap<WBITS, WG, MODE><<<n, WG, 0, st>>>(s.g, perm, o);
+        return;
+    }
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_bitmap_persist<WBITS, WG, MODE>, WG, 0) !=
+            hipSuccess ||
+        per_cu < 1) {
+        (void)hipGetLastError();
+        per_cu = 1;
+    }
+    const u64 fit = (u64)ctl->cu_count * (u64)per_cu;
+    u32 *ctr = s.ctr.ptr + MODE;  // one counter per pass of the SpGEMM
+    fill_u32_on(st, ctr, 1, 0u);
+    k_bitmap_persist<WBITS, WG, MODE><<<(u32)(n < fit ? n : fit), WG, 0, st>>>(s.g, perm, n, ctr, o);
 }
 
 void launch_bins(clb_controls_s *ctl, const Setup &s, const u32 *perm, const u32 off[NBINS + 1], int mode,
@@ -403,9 +424,9 @@ void launch_bins(clb_controls_s *ctl, const Setup &s, const u32 *perm, const u32
             t.begin(st);
 #define BITMAP3(WB, WGS)                                                   \
     do {                                                                   \
-        if (mode == SYM) launch_bitmap<WB, WGS, SYM>(st, s.g, pp, n, o); \
-        else if (mode == NUM) launch_bitmap<WB, WGS, NUM>(st, s.g, pp, n, o); \
-        else launch_bitmap<WB, WGS, ONE>(st, s.g, pp, n, o);          \
+        if (mode == SYM) launch_bitmap<WB, WGS, SYM>(ctl, st, s, pp, n, o); \
+        else if (mode == NUM) launch_bitmap<WB, WGS, NUM>(ctl, st, s, pp, n, o); \
+        else launch_bitmap<WB, WGS, ONE>(ctl, st, s, pp, n, o);          \
     } while (0)
             switch (s.cfg.bm_bits) {
                 case 6: BITMAP3(6, 64); break;

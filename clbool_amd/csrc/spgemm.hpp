@@ -14,6 +14,7 @@ struct Setup {
     DevArray<uint2> brange;
     DevArray<u32> prod;
     DevArray<u64> scal;
+    DevArray<u32> ctr;  // row counters of persistent launches (zeroed per launch)
     Mats g{};
     BinCfg cfg{};
 };

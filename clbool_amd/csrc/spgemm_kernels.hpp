@@ -690,5 +690,30 @@ __global__ __launch_bounds__(WG) void k_bitmap(Mats g, const u32 *__restrict__ p
     bitmap_row<WBITS, WG, MODE>(g, perm, blockIdx.x, o, bm, big, red);
 }
 
+// The same rows, persistent: a grid of the workgroups that fit on the device
+// at once, each taking its next row from the counter *ctr (zero at launch)
+// as soon as it finishes one -- dynamic balance over rows of very different
+// sizes without a dispatch per row.  The next row is claimed at the start of
+// the current one.
+template <int WBITS, int WG, int MODE>
+__global__ __launch_bounds__(WG) void k_bitmap_persist(Mats g, const u32 *__restrict__ perm, u32 count,
+                                                       u32 *__restrict__ ctr, Out o) {
+    __shared__ __align__(16) u32 bm[1u << WBITS];
+    __shared__ __align__(8) u32 big[BIG_WORDS(WG)];
+    __shared__ u32 red[WG / 64 + 1];
+    __shared__ u32 next;
+    if (threadIdx.x == 0) next = atomicAdd(ctr, 1u);
+    __syncthreads();
+    u32 idx = next;
+    __syncthreads();
+    while (idx < count) {
+        if (threadIdx.x == 0) next = atomicAdd(ctr, 1u);
+        bitmap_row<WBITS, WG, MODE>(g, perm, idx, o, bm, big, red);
+        __syncthreads();
+        idx = next;
+        __syncthreads();
+    }
+}
+
 }  // namespace spg
 }  // namespace clb

@@ -76,7 +76,10 @@ typedef struct clb_tuning {
                                       at least this many set bits per 1024 words
                                       are dumped and expanded by a separate
                                       emission kernel (0: 256; >= 65536: off)     */
-    uint32_t reserved[9];
+    uint32_t bitmap_persistent;    /* nonzero: the bitmap rows run on a persistent
+                                      grid that takes rows from a counter
+                                      instead of one workgroup per row          */
+    uint32_t reserved[8];
 } clb_tuning;
 
 #define CLB_SPGEMM_PLAIN   0u  /* two-pass: symbolic count, scan, numeric      */

@@ -94,6 +94,9 @@ def test_hypersparse_2p24(controls):
     dict(hash_max_slots=16384, bitmap_min_products=1 << 30, bitmap_slot_ratio=1, tiny_max_products=64),
     dict(hash_max_slots=16384, bitmap_min_products=1, bitmap_slot_ratio=1, tiny_max_products=0),
     dict(hash_max_slots=1024, bitmap_min_products=4096, bitmap_slot_ratio=4, tiny_max_products=16),
+    dict(hash_max_slots=16384, bitmap_min_products=1, bitmap_slot_ratio=1, tiny_max_products=0, bitmap_persistent=1),
+    dict(hash_max_slots=1024, bitmap_min_products=4096, bitmap_slot_ratio=4, tiny_max_products=16,
+         bitmap_persistent=1),
 ])
 def test_every_kernel_path(controls, knobs):
     """Force the rows through each bin family (hash only, tiny, bitmap only)."""
@@ -191,8 +194,9 @@ def _hub_product(seed, nrows=1200, hubs=64, ncols=1 << 20):
     return (ra, ca, nrows, hubs), (rb, cbv, hubs, ncols)
 
 
+@pytest.mark.parametrize("persistent", [0, 1])
 @pytest.mark.parametrize("dense_knob", [16, 256, 65536])
-def test_dense_bitmap_rows(controls, dense_knob):
+def test_dense_bitmap_rows(controls, dense_knob, persistent):
     """Rows over a 2^20-column window: dumped dense bitmaps (expanded by the
     emission kernel) and in-LDS emission must give the same exact result."""
     (ra, ca, m, k), (rb, cbv, k2, n) = _hub_product(5)
@@ -202,14 +206,15 @@ def test_dense_bitmap_rows(controls, dense_knob):
     old = controls.tuning()
     t = controls.tuning()
     t.dump_bits_per_chunk = dense_knob
+    t.bitmap_persistent = persistent
     controls.set_tuning(t)
     try:
         got, st = cb.dcsr.spgemm(controls, a, b, timing=True)
-        assert_dcsr_equal(controls, got, ref, f"plain dense_knob={dense_knob}")
+        assert_dcsr_equal(controls, got, ref, f"plain dense_knob={dense_knob} persistent={persistent}")
         emits = st["launches"][4 * 1 + 3]
         assert emits == (0 if dense_knob == 65536 else 1)
         cb.predictor_reset(controls, 3)
         got, st = cb.dcsr.spgemm(controls, a, b, mode=cb.SPGEMM_LEARNED, timing=True)
-        assert_dcsr_equal(controls, got, ref, f"learned dense_knob={dense_knob}")
+        assert_dcsr_equal(controls, got, ref, f"learned dense_knob={dense_knob} persistent={persistent}")
     finally:
         controls.set_tuning(old)
