@@ -298,7 +298,7 @@ def main():
             tuned[m] = loaded[m]
             log(f"[bench] {m}: tuning from {args.tuning_file}: {tuned[m]}")
         elif not args.no_autotune:
-            sample = blocks[::max(1, len(blocks) // 3)][:4]
+            sample = blocks
             res = at.autotune(ctl, a_dev, a_dev, sample, mode=code[m], repeats=2, log=log)
             tuned[m] = res["tuning"]
             log(f"[bench] {m}: autotuned {res['tuning']} over {res['evaluated']} settings")
