@@ -192,6 +192,69 @@ def bench_closure(cb, gen, ctl, max_iters, log, seed=0):
     return out
 
 
+def bench_hypersparse(cb, gen, ctl, n_train, log, n_test=4, log_n=24):
+    """The hypersparse learned-sizing workload: 2^24 x 2^24, 1% of rows
+    nonempty, Zipf(1.5) row lengths capped at 4096, uniform columns; A * A.
+    The predictor is trained on exact row sizes of n_train seeded instances
+    (none of them a test instance), then plain and learned SpGEMMs of n_test
+    other instances are compared: identical results, fallback rate, and the
+    error of the predicted row sizes."""
+    n = 1 << log_n
+    t0 = time.perf_counter()
+    cb.predictor_reset(ctl, seed=1)
+    total, nnz_seen = 0, []
+    for seed in range(1, n_train + 1):
+        r, c = gen.hypersparse_zipf(seed=seed, log_n=log_n, row_frac=0.01)
+        rpt, rows, cols = gen.to_dcsr(r, c)
+        nnz_seen.append(int(cols.size))
+        m = cb.matrix_dcsr_from_cpu(ctl, rpt, rows, cols, n, n)
+        total = cb.predictor_harvest(ctl, m, m, max_rows=1 << 24)
+        del m
+    gen_s = time.perf_counter() - t0
+    tr = cb.predictor_train(ctl, epochs=20, batch=4096, lr=3e-3, seed=1)
+    log(f"[bench] hypersparse: {n_train} instances ({np.mean(nnz_seen):.3g} nnz each), {total} samples, "
+        f"loss {tr['loss_first']:.5f} -> {tr['loss_last']:.6f}")
+    tests = []
+    for seed in range(100001, 100001 + n_test):
+        r, c = gen.hypersparse_zipf(seed=seed, log_n=log_n, row_frac=0.01)
+        rpt, rows, cols = gen.to_dcsr(r, c)
+        a = cb.matrix_dcsr_from_cpu(ctl, rpt, rows, cols, n, n)
+        res = {}
+        for name, mode in (("plain", cb.SPGEMM_PLAIN), ("learned", cb.SPGEMM_LEARNED)):
+            C, st = cb.dcsr.spgemm(ctl, a, a, mode=mode)  # warm
+            ts = []
+            for _ in range(5):
+                t1 = time.perf_counter()
+                C, st = cb.dcsr.spgemm(ctl, a, a, mode=mode)
+                ctl.sync()
+                ts.append(time.perf_counter() - t1)
+            res[name] = (C, st, min(ts))
+        Cp, stp, tp = res["plain"]
+        Cl, stl, tl = res["learned"]
+        same = _fingerprint(ctl, Cp) == _fingerprint(ctl, Cl)
+        # predicted against exact row sizes, per stored row of A
+        pred = cb.predictor_predict(ctl, a, a).astype(np.float64)
+        exact = np.zeros(rows.size, np.float64)
+        if Cp.nnz:
+            crow = Cp.rows_gpu.read(ctl, Cp.nzr)
+            crpt = Cp.rpt_gpu.read(ctl, Cp.nzr + 1).astype(np.int64)
+            exact[np.searchsorted(rows, crow)] = np.diff(crpt)
+        nz = exact > 0
+        tests.append({"seed": seed, "a_nnz": int(cols.size), "a_nzr": int(rows.size), "products": int(stp["products"]),
+                      "c_nnz": int(Cp.nnz), "identical": bool(same),
+                      "fallback_rows": int(stl["fallback_rows"]), "fallback_rate": stl["fallback_rows"] / max(1, rows.size),
+                      "mae_rows": float(np.abs(pred - exact).mean()),
+                      "mape_rows": float((np.abs(pred - exact)[nz] / exact[nz]).mean()) if nz.any() else 0.0,
+                      "mean_abs_log2_err": float(np.abs(np.log2(1 + np.maximum(pred, 0)) - np.log2(1 + exact)).mean()),
+                      "capacity_over_nnz": stl["capacity"] / max(1, Cp.nnz),
+                      "plain_ms": tp * 1e3, "learned_ms": tl * 1e3})
+        log(f"[bench] hypersparse test {seed}: identical={same} fallback={stl['fallback_rows']} "
+            f"mae={tests[-1]['mae_rows']:.3f} plain {tp * 1e3:.2f} ms learned {tl * 1e3:.2f} ms")
+        del a, Cp, Cl
+    return {"train_instances": n_train, "train_samples": int(total), "train_nnz_mean": float(np.mean(nnz_seen)),
+            "generation_and_harvest_s": gen_s, "training": tr, "tests": tests}
+
+
 def cpu_baseline(oracle, A, target_s=12.0, seed=7):
     """matrix_multiplication_cpu (coo_utils.cpp:470-514) restated in oracle/,
     single thread, on a bounded uniform sample of A's rows times the full B."""
@@ -244,6 +307,10 @@ def main():
     ap.add_argument("--tuning-file", default=None,
                     help="JSON written by a previous run (--save-tuning): use its knobs instead of tuning")
     ap.add_argument("--save-tuning", default=None)
+    ap.add_argument("--hypersparse", type=int, default=0,
+                    help="also run the hypersparse learned-sizing workload, training on this many instances")
+    ap.add_argument("--hypersparse-only", type=int, default=0,
+                    help="run only the hypersparse workload, training on this many instances")
     ap.add_argument("--copy-probe", type=float, default=0,
                     help="only run a device-to-device copy of this many bytes (4 times): the reference "
                          "point that scales the FETCH_SIZE / WRITE_SIZE counters of a profiled run")
@@ -261,6 +328,12 @@ def main():
             dt = time.perf_counter() - t0
             log(f"[copy-probe] {nbytes} bytes in {dt * 1e3:.3f} ms ({2 * nbytes / dt / 1e9:.1f} GB/s read+write)")
         print(json.dumps({"copy_probe_bytes": nbytes}))
+        return
+    if args.hypersparse_only:
+        import clbool_amd as cb
+        from clbool_amd import generators as gen
+        ctl = cb.create_controls()
+        print(json.dumps({"hypersparse": bench_hypersparse(cb, gen, ctl, args.hypersparse_only, log)}), flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU row-block sharding is a "
@@ -386,6 +459,8 @@ def main():
     if args.closure_iters > 0 and "learned" in modes:
         at.apply(ctl, tuned.get("learned", base_tuning))
         out["closure"] = bench_closure(cb, gen, ctl, args.closure_iters, log)
+    if args.hypersparse > 0:
+        out["hypersparse"] = bench_hypersparse(cb, gen, ctl, args.hypersparse, log)
     if not args.no_cpu_baseline:
         log("[bench] timing the CPU baseline")
         out["cpu_baseline"] = cpu_baseline(oracle, A)
