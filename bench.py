@@ -35,6 +35,7 @@ HBM_PEAK_GBS = 8000.0
 WORKLOADS = {
     # name: (scale, edge_factor, seed)
     "rmat20": (20, 16, 1),
+    "rmat22": (22, 16, 1),
     "rmat18": (18, 16, 1),
     "rmat16": (16, 16, 1),
 }

@@ -16,6 +16,32 @@
 namespace clb {
 namespace spg {
 
+// Mats::splits of B: for each stored row, the start of each column slice of
+// SPAN columns (lower bounds in the sorted row) and its end.
+__global__ void k_slice_splits(const u32 *__restrict__ b_rpt, const u32 *__restrict__ b_rows,
+                               const u32 *__restrict__ b_cols, u32 b_nzr, u32 mapn, u32 nsl, u64 span,
+                               u32 *__restrict__ splits) {
+    const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= b_nzr) return;
+    const u32 k = b_rows[i];
+    if (k >= mapn) return;
+    const u32 beg = b_rpt[i], end = b_rpt[i + 1];
+    u32 *out = splits + (u64)k * (nsl + 1);
+    u32 lo = beg;
+    out[0] = beg;
+    for (u32 s = 1; s < nsl; ++s) {
+        const u64 bound = (u64)s * span;
+        u32 hi = end;
+        while (lo < hi) {  // first column >= bound, searching on from the previous slice
+            const u32 mid = lo + (hi - lo) / 2;
+            if ((u64)b_cols[mid] < bound) lo = mid + 1;
+            else hi = mid;
+        }
+        out[s] = lo;
+    }
+    out[nsl] = end;
+}
+
 __global__ void k_brange_set(const u32 *__restrict__ b_rpt, const u32 *__restrict__ b_rows, u32 b_nzr, u32 mapn,
                              uint2 *__restrict__ brange) {
     const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -228,6 +254,19 @@ void prepare(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &b, u32 row_
     s.g.ncols = b.ncols;
     s.g.emit = ctl->tuning.emit_lane_bits_min;
     s.cfg = make_bincfg(ctl, b.ncols);
+    const u64 span = 32ull << s.cfg.bm_bits;
+    const u64 nsl = ((u64)b.ncols + span - 1) / span;
+    s.g.splits = nullptr;
+    s.g.nsl = 0;
+    if (nsl > 1 && b.nzr && (u64)s.mapn * (nsl + 1) < (1ull << 32)) {
+        s.splits.reset(ctl, (u64)s.mapn * (nsl + 1));
+        fill_u32(ctl, s.splits.ptr, s.splits.n, 0u);
+        k_slice_splits<<<div_up(b.nzr, 256), 256, 0, ctl->stream>>>(u32p(b.rpt), u32p(b.rows), u32p(b.cols), b.nzr,
+                                                                     s.mapn, (u32)nsl, span, s.splits.ptr);
+        CLB_LAUNCHED("slice_splits");
+        s.g.splits = s.splits.ptr;
+        s.g.nsl = (u32)nsl;
+    }
     s.prod.reset(ctl, nrows);
     s.ctr.reset(ctl, 4);
     s.scal.reset(ctl, 8);  // [0] total products, [1] nnz total, [2] nzr total, [3] overflow flag

@@ -15,6 +15,7 @@ struct Setup {
     DevArray<u32> prod;
     DevArray<u64> scal;
     DevArray<u32> ctr;  // row counters of persistent launches (zeroed per launch)
+    DevArray<u32> splits;  // per-slice B row ranges (Mats::splits) when ncols spans several windows
     Mats g{};
     BinCfg cfg{};
 };

@@ -46,7 +46,14 @@ struct Mats {
     u32 row0;             // first A stored row of the block
     u32 ncols;            // columns of B (and C)
     u32 emit;             // bitmap emission: lane-per-bit threshold (EMIT_DEFAULT when 0)
+    // Column slices of the bitmap kernels (nsl > 1 when ncols exceeds one
+    // bitmap window): splits[k * (nsl + 1) + s] is the position in b_cols of
+    // B row k's first column in slice s (s = nsl: the row's end), so a slice
+    // reads only its own part of each B row.
+    const u32 *splits;
+    u32 nsl;
 };
+constexpr u32 NOSLICE = 0xFFFFFFFFu;
 
 // Bitmap emission.  Each wave owns a contiguous range of the bitmap's words
 // and walks it 64 words at a time (dev::emit_step): a step whose 64 words
@@ -105,7 +112,8 @@ constexpr int UNROLL = 4;
 __host__ __device__ constexpr int BIG_WORDS(int WG) { return 3 * WG + 2 * (WG / 64 + 2); }
 
 template <int WG, int UB, u32 BIGMIN, typename F>
-__device__ __forceinline__ void for_each_product(const Mats &g, u32 js, u32 je, u32 *big, F &&f) {
+__device__ __forceinline__ void for_each_product(const Mats &g, u32 js, u32 je, u32 *big, F &&f,
+                                                 u32 slice = NOSLICE) {
     const int l = dev::lane();
     constexpr int NW = WG / dev::WAVE;
     constexpr u32 WIN = UB * 64;
@@ -115,9 +123,15 @@ __device__ __forceinline__ void for_each_product(const Mats &g, u32 js, u32 je, 
         if (j < je) {
             const u32 k = g.a_cols[j];
             if (k < g.mapn) {
-                const uint2 r = g.brange[k];
-                bx = r.x;
-                len = r.y - r.x;
+                if (slice != NOSLICE) {
+                    const u32 *sp = g.splits + (u64)k * (g.nsl + 1) + slice;
+                    bx = sp[0];
+                    len = sp[1] - bx;
+                } else {
+                    const uint2 r = g.brange[k];
+                    bx = r.x;
+                    len = r.y - r.x;
+                }
             }
         }
         // list entry e: big[3e] = first B position, big[3e+1] = end,
@@ -585,11 +599,14 @@ __device__ __forceinline__ void bitmap_row(const Mats &g, const u32 *__restrict_
         if (nslices == 1)
             for_each_product<WG, (MODE == SYM ? 16 : 8), LONG_ROW>(
                 g, js, je, big, [&](u32 col) { atomicOr(&bm[col >> 5], 1u << (col & 31)); });
-        else
-            for_each_product<WG, (MODE == SYM ? 16 : 8), LONG_ROW>(g, js, je, big, [&](u32 col) {
-                const u64 d = (u64)col - base;
-                if (d < SPAN) atomicOr(&bm[(u32)(d >> 5)], 1u << (col & 31));
-            });
+        else  // only the slice's part of each B row is read
+            for_each_product<WG, (MODE == SYM ? 16 : 8), LONG_ROW>(
+                g, js, je, big,
+                [&](u32 col) {
+                    const u64 d = (u64)col - base;
+                    if (d < SPAN) atomicOr(&bm[(u32)(d >> 5)], 1u << (col & 31));
+                },
+                g.nsl == nslices ? s : NOSLICE);
         __syncthreads();
         if (MODE != SYM && WBITS >= 14 && nslices == 1 && o.dump.store) {
             // exact count per 1024-word chunk; dense rows are dumped whole
