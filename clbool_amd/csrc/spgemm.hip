@@ -172,8 +172,9 @@ void setup_dump(clb_controls_s *ctl, const Setup &s, const BinHost &h, DumpBufs 
     if (s.cfg.bm_bits < 14 || knob >= 65536u) return;
     const u64 nslices = ((u64)s.g.ncols + (32ull << s.cfg.bm_bits) - 1) / (32ull << s.cfg.bm_bits);
     d.nbm = h.off[NBINS] - h.off[BIN_BITMAP0];
-    if (nslices != 1 || d.nbm == 0) return;
-    d.words = 1u << s.cfg.bm_bits;
+    // a dumped row keeps every slice: nslices windows of 2^bm_bits words
+    if (d.nbm == 0 || (nslices << s.cfg.bm_bits) > (1ull << 26)) return;
+    d.words = (u32)(nslices << s.cfg.bm_bits);
     d.nch = d.words / EMIT_CHUNK;
     const u64 bytes = (u64)d.nbm * d.words * 4;
     size_t free_b = 0, total_b = 0;
@@ -190,7 +191,9 @@ void setup_dump(clb_controls_s *ctl, const Setup &s, const BinHost &h, DumpBufs 
     fill_u32(ctl, d.cnt.ptr, d.cnt.n, 0);
     fill_u32(ctl, d.count.ptr, 1, 0);
     fill_u32(ctl, d.isdense.ptr, s.nrows, 0);
-    o.dump = Dump{d.store.ptr, d.cnt.ptr, d.list.ptr, d.count.ptr, d.isdense.ptr, (d.words / 1024u) * knob};
+    // the decision is taken on slice 0's count
+    o.dump = Dump{d.store.ptr, d.cnt.ptr, d.list.ptr, d.count.ptr, d.isdense.ptr,
+                  ((1u << s.cfg.bm_bits) / 1024u) * knob};
     d.on = true;
 }
 

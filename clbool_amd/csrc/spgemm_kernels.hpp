@@ -587,6 +587,7 @@ __device__ __forceinline__ void bitmap_row(const Mats &g, const u32 *__restrict_
     const u32 nslices = (u32)(((u64)g.ncols + SPAN - 1) / SPAN);
     u32 mine = 0;
     u64 written = 0;
+    bool dumping = false;  // uniform: the row goes to the dense-row emission
     u32 *out = MODE == SYM ? nullptr : o.cols + o.off[r];
     const u64 lim = row_limit<MODE>(o, r);
     for (u32 s = 0; s < nslices; ++s) {
@@ -608,34 +609,33 @@ __device__ __forceinline__ void bitmap_row(const Mats &g, const u32 *__restrict_
                 },
                 g.nsl == nslices ? s : NOSLICE);
         __syncthreads();
-        if (MODE != SYM && WBITS >= 14 && nslices == 1 && o.dump.store) {
-            // exact count per 1024-word chunk; dense rows are dumped whole
-            constexpr u32 WPT = W / WG, TPC = EMIT_CHUNK / WPT, NCH = W / EMIT_CHUNK;
+        if (MODE != SYM && WBITS >= 14 && o.dump.store) {
+            // exact count per 1024-word chunk.  Slice 0 decides for the whole
+            // row: a dense row has every slice dumped (slice s at word s * W
+            // of its slot) and is written later by the emission kernel
+            constexpr u32 WPT = W / WG, TPC = EMIT_CHUNK / WPT, NCH = W / EMIT_CHUNK, Q = WPT / 4;
             const uint4 *bm4 = reinterpret_cast<const uint4 *>(bm);
             u32 cnt = 0;
 #pragma unroll 2
-            for (u32 q = 0; q < WPT / 4; ++q) {
-                const uint4 v = bm4[threadIdx.x * (WPT / 4) + q];
+            for (u32 q = 0; q < Q; ++q) {
+                // rotated start: neighbouring threads read different banks
+                const uint4 v = bm4[threadIdx.x * Q + ((q + threadIdx.x) & (Q - 1))];
                 cnt += __popc(v.x) + __popc(v.y) + __popc(v.z) + __popc(v.w);
             }
             u32 csum = cnt;
 #pragma unroll
             for (u32 off2 = TPC / 2; off2 > 0; off2 >>= 1) csum += __shfl_xor(csum, off2, TPC);
             const u32 n = dev::block_sum<u32, WG>(cnt, red);
-            // numeric pass: only a row whose count matches the symbolic pass
-            // (its slot size) is dumped; anything else takes the capped path below
-            if (n >= o.dump.dense_min && (MODE == ONE ? n <= lim : n == lim)) {
+            if (s == 0) dumping = n >= o.dump.dense_min;
+            if (dumping) {
                 const u32 slot = idx;
-                uint4 *dst = o.dump.store + (u64)slot * (W / 4);
+                uint4 *dst = o.dump.store + ((u64)slot * nslices + s) * (W / 4);
                 for (u32 q = threadIdx.x; q < W / 4; q += WG) dst[q] = bm4[q];
-                if (threadIdx.x % TPC == 0) o.dump.chunk_cnt[(u64)slot * NCH + threadIdx.x / TPC] = csum;
-                if (threadIdx.x == 0) {
-                    o.dump.list[atomicAdd(o.dump.count, 1u)] = slot;
-                    o.dump.isdense[r] = 1;
-                    if (MODE == ONE) o.nnz_row[r] = n;
-                    else if (n != lim) atomicOr(o.overflow, 2u);
-                }
-                return;
+                if (threadIdx.x % TPC == 0)
+                    o.dump.chunk_cnt[((u64)slot * nslices + s) * NCH + threadIdx.x / TPC] = csum;
+                written += n;
+                __syncthreads();  // before the next slice clears the bitmap
+                continue;
             }
         }
         if (MODE == SYM) {
@@ -685,6 +685,22 @@ __device__ __forceinline__ void bitmap_row(const Mats &g, const u32 *__restrict_
     if (MODE == SYM) {
         const u32 n = dev::block_sum<u32, WG>(mine, red);
         if (threadIdx.x == 0) o.nnz_row[r] = n;
+    } else if (dumping) {
+        // listed for emission only when it fits its slot: the numeric pass
+        // needs the symbolic count, the one pass its capacity
+        if (threadIdx.x == 0) {
+            const bool fits = MODE == ONE ? written <= lim : written == lim;
+            if (fits) {
+                o.dump.list[atomicAdd(o.dump.count, 1u)] = idx;
+                o.dump.isdense[r] = 1;
+                if (MODE == ONE) o.nnz_row[r] = (u32)written;
+            } else if (MODE == ONE) {
+                o.nnz_row[r] = FAILED;
+                atomicAdd(o.overflow, 1u);
+            } else {
+                atomicOr(o.overflow, 2u);
+            }
+        }
     } else if (threadIdx.x == 0) {
         if (MODE == ONE) {
             // the capacity was too small: the partial output is discarded

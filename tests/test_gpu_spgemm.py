@@ -218,3 +218,28 @@ def test_dense_bitmap_rows(controls, dense_knob, persistent):
         assert_dcsr_equal(controls, got, ref, f"learned dense_knob={dense_knob} persistent={persistent}")
     finally:
         controls.set_tuning(old)
+
+
+@pytest.mark.parametrize("dense_knob", [4, 64, 65536])
+def test_dense_rows_over_column_slices(controls, dense_knob):
+    """2^22 columns (four bitmap windows): rows whose first slice is dense are
+    dumped slice by slice and expanded by the emission kernel; the others are
+    emitted slice by slice in LDS.  Both passes and the learned pass must agree
+    with the oracle."""
+    (ra, ca, m, k), (rb, cbv, k2, n) = _hub_product(6, nrows=600, ncols=1 << 22)
+    A, a = pairs_dcsr(controls, ra, ca, m, k)
+    B, b = pairs_dcsr(controls, rb, cbv, k2, n)
+    ref = oracle.spgemm(A, B)
+    old = controls.tuning()
+    t = controls.tuning()
+    t.dump_bits_per_chunk = dense_knob
+    controls.set_tuning(t)
+    try:
+        got, st = cb.dcsr.spgemm(controls, a, b, timing=True)
+        assert_dcsr_equal(controls, got, ref, f"plain sliced dense_knob={dense_knob}")
+        assert (st["dense_rows"] > 0) == (dense_knob != 65536)
+        cb.predictor_reset(controls, 4)
+        got, st = cb.dcsr.spgemm(controls, a, b, mode=cb.SPGEMM_LEARNED, timing=True)
+        assert_dcsr_equal(controls, got, ref, f"learned sliced dense_knob={dense_knob}")
+    finally:
+        controls.set_tuning(old)
