@@ -237,12 +237,13 @@ __device__ __forceinline__ void for_each_product(const Mats &g, u32 js, u32 je, 
 #pragma unroll
                     for (int u = 0; u < UB; ++u) c[u] = src[u * 64 + l];
                 } else {
-                    // the row's last window: lanes past its end re-read its
-                    // last column and are masked off when applied
+                    // the row's last window: only its pieces are loaded; lanes
+                    // past its end re-read its last column and are masked off
+                    // when applied
 #pragma unroll
                     for (int u = 0; u < UB; ++u) {
                         const u32 o = (u32)(u * 64 + l);
-                        c[u] = src[o < n ? o : n - 1];
+                        if ((u32)(u * 64) < n) c[u] = src[o < n ? o : n - 1];
                     }
                 }
                 return n;
