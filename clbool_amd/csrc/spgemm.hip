@@ -228,10 +228,17 @@ BinCfg make_bincfg(const clb_controls_s *ctl, u32 ncols) {
     c.hash_max_bits = hb < HASH_MIN_BITS ? HASH_MIN_BITS : (hb > HASH_MAX_BITS ? HASH_MAX_BITS : hb);
     c.bitmap_min = ctl->tuning.bitmap_min_products;
     c.bitmap_ratio = ctl->tuning.bitmap_slot_ratio;
+    // Window of a bitmap kernel: the matrix width rounded up to a power of
+    // two words, at most 2^15 words (128 KB of LDS, one 1024-thread workgroup
+    // per CU) for the symbolic count and at most 2^14 (64 KB, two 512-thread
+    // workgroups per CU) for the kernels that also write columns: their
+    // emission phase then overlaps the other workgroup's gather.  Wider
+    // matrices are processed in column slices.
     const u64 words = ((u64)ncols + 31) / 32;
     u32 wb = 6;
     while (wb < 15 && (1ull << wb) < words) ++wb;
-    c.bm_bits = wb;
+    c.sym_bits = wb;
+    c.bm_bits = wb < 14 ? wb : 14;
     return c;
 }
 
@@ -261,6 +268,7 @@ void prepare(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &b, u32 row_
     const u64 nsl = ((u64)b.ncols + span - 1) / span;
     s.g.splits = nullptr;
     s.g.nsl = 0;
+    s.g.split_bits = s.cfg.bm_bits;
     if (nsl > 1 && b.nzr && (u64)s.mapn * (nsl + 1) < (1ull << 32)) {
         s.splits.reset(ctl, (u64)s.mapn * (nsl + 1));
         fill_u32(ctl, s.splits.ptr, s.splits.n, 0u);
@@ -470,7 +478,7 @@ void launch_bins(clb_controls_s *ctl, const Setup &s, const u32 *perm, const u32
         else if (mode == NUM) launch_bitmap<WB, WGS, NUM>(ctl, st, s, pp, n, o); \
         else launch_bitmap<WB, WGS, ONE>(ctl, st, s, pp, n, o);          \
     } while (0)
-            switch (s.cfg.bm_bits) {
+            switch (mode == SYM ? s.cfg.sym_bits : s.cfg.bm_bits) {
                 case 6: BITMAP3(6, 64); break;
                 case 7: BITMAP3(7, 64); break;
                 case 8: BITMAP3(8, 64); break;
@@ -479,8 +487,8 @@ void launch_bins(clb_controls_s *ctl, const Setup &s, const u32 *perm, const u32
                 case 11: BITMAP3(11, 256); break;
                 case 12: BITMAP3(12, 512); break;
                 case 13: BITMAP3(13, 512); break;
-                case 14: BITMAP3(14, 1024); break;
-                case 15: BITMAP3(15, 1024); break;
+                case 14: BITMAP3(14, 512); break;
+                case 15: launch_bitmap<15, 1024, SYM>(ctl, st, s, pp, n, o); break;  // symbolic only
             }
 #undef BITMAP3
             CLB_LAUNCHED("bitmap");

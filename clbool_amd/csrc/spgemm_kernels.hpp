@@ -47,11 +47,13 @@ struct Mats {
     u32 ncols;            // columns of B (and C)
     u32 emit;             // bitmap emission: lane-per-bit threshold (EMIT_DEFAULT when 0)
     // Column slices of the bitmap kernels (nsl > 1 when ncols exceeds one
-    // bitmap window): splits[k * (nsl + 1) + s] is the position in b_cols of
-    // B row k's first column in slice s (s = nsl: the row's end), so a slice
-    // reads only its own part of each B row.
+    // window of 2^split_bits words): splits[k * (nsl + 1) + s] is the
+    // position in b_cols of B row k's first column in slice s (s = nsl: the
+    // row's end), so a slice reads only its own part of each B row.  A kernel
+    // with a window 2^d times as wide takes every 2^d-th boundary.
     const u32 *splits;
     u32 nsl;
+    u32 split_bits;
 };
 constexpr u32 NOSLICE = 0xFFFFFFFFu;
 
@@ -67,7 +69,8 @@ struct BinCfg {
     u32 hash_max_bits;
     u32 bitmap_min;
     u32 bitmap_ratio;
-    u32 bm_bits;  // log2 of the bitmap window in 32-bit words
+    u32 bm_bits;   // log2 of the bitmap window in 32-bit words (numeric and one-pass kernels)
+    u32 sym_bits;  // the same for the symbolic kernel
 };
 
 __device__ __forceinline__ u32 ceil_log2(u32 x) { return x <= 1 ? 0 : 32 - __clz(x - 1); }
@@ -113,7 +116,7 @@ __host__ __device__ constexpr int BIG_WORDS(int WG) { return 3 * WG + 2 * (WG / 
 
 template <int WG, int UB, u32 BIGMIN, typename F>
 __device__ __forceinline__ void for_each_product(const Mats &g, u32 js, u32 je, u32 *big, F &&f,
-                                                 u32 slice = NOSLICE) {
+                                                 u32 slice = NOSLICE, u32 sstep = 1) {
     const int l = dev::lane();
     constexpr int NW = WG / dev::WAVE;
     constexpr u32 WIN = UB * 64;
@@ -124,9 +127,11 @@ __device__ __forceinline__ void for_each_product(const Mats &g, u32 js, u32 je, 
             const u32 k = g.a_cols[j];
             if (k < g.mapn) {
                 if (slice != NOSLICE) {
-                    const u32 *sp = g.splits + (u64)k * (g.nsl + 1) + slice;
-                    bx = sp[0];
-                    len = sp[1] - bx;
+                    // fine slices [slice * sstep, (slice + 1) * sstep) of the row
+                    const u32 *sp = g.splits + (u64)k * (g.nsl + 1);
+                    const u32 f0 = slice * sstep, f1 = f0 + sstep < g.nsl ? f0 + sstep : g.nsl;
+                    bx = sp[f0];
+                    len = sp[f1] - bx;
                 } else {
                     const uint2 r = g.brange[k];
                     bx = r.x;
@@ -586,6 +591,9 @@ __device__ __forceinline__ void bitmap_row(const Mats &g, const u32 *__restrict_
     const u32 i = g.row0 + r;
     const u32 js = g.a_rpt[i], je = g.a_rpt[i + 1];
     const u32 nslices = (u32)(((u64)g.ncols + SPAN - 1) / SPAN);
+    // the split table's slices are this window or a power-of-two fraction of it
+    const bool use_splits = g.splits && g.nsl > 1 && (u32)WBITS >= g.split_bits;
+    const u32 sstep = use_splits ? 1u << (WBITS - g.split_bits) : 1u;
     u32 mine = 0;
     u64 written = 0;
     bool dumping = false;  // uniform: the row goes to the dense-row emission
@@ -608,7 +616,7 @@ __device__ __forceinline__ void bitmap_row(const Mats &g, const u32 *__restrict_
                     const u64 d = (u64)col - base;
                     if (d < SPAN) atomicOr(&bm[(u32)(d >> 5)], 1u << (col & 31));
                 },
-                g.nsl == nslices ? s : NOSLICE);
+                use_splits ? s : NOSLICE, sstep);
         __syncthreads();
         if (MODE != SYM && WBITS >= 14 && o.dump.store) {
             // exact count per 1024-word chunk.  Slice 0 decides for the whole

@@ -197,8 +197,9 @@ def _hub_product(seed, nrows=1200, hubs=64, ncols=1 << 20):
 @pytest.mark.parametrize("persistent", [0, 1])
 @pytest.mark.parametrize("dense_knob", [16, 256, 65536])
 def test_dense_bitmap_rows(controls, dense_knob, persistent):
-    """Rows over a 2^20-column window: dumped dense bitmaps (expanded by the
-    emission kernel) and in-LDS emission must give the same exact result."""
+    """Rows over 2^20 columns (one symbolic bitmap window, two numeric ones):
+    dumped dense bitmaps (expanded by the emission kernel) and in-LDS emission
+    must give the same exact result."""
     (ra, ca, m, k), (rb, cbv, k2, n) = _hub_product(5)
     A, a = pairs_dcsr(controls, ra, ca, m, k)
     B, b = pairs_dcsr(controls, rb, cbv, k2, n)
@@ -220,13 +221,16 @@ def test_dense_bitmap_rows(controls, dense_knob, persistent):
         controls.set_tuning(old)
 
 
+@pytest.mark.parametrize("ncols", [1 << 22, (1 << 20) + 12345])
 @pytest.mark.parametrize("dense_knob", [4, 64, 65536])
-def test_dense_rows_over_column_slices(controls, dense_knob):
-    """2^22 columns (four bitmap windows): rows whose first slice is dense are
-    dumped slice by slice and expanded by the emission kernel; the others are
-    emitted slice by slice in LDS.  Both passes and the learned pass must agree
-    with the oracle."""
-    (ra, ca, m, k), (rb, cbv, k2, n) = _hub_product(6, nrows=600, ncols=1 << 22)
+def test_dense_rows_over_column_slices(controls, dense_knob, ncols):
+    """Columns over several bitmap windows (2^22: four symbolic windows of
+    2^20 columns and eight numeric ones of 2^19; 2^20 + 12345: a partial last
+    window, two symbolic and three numeric): rows whose first slice is dense
+    are dumped slice by slice and expanded by the emission kernel; the others
+    are emitted slice by slice in LDS.  Both passes and the learned pass must
+    agree with the oracle."""
+    (ra, ca, m, k), (rb, cbv, k2, n) = _hub_product(6, nrows=600, ncols=ncols)
     A, a = pairs_dcsr(controls, ra, ca, m, k)
     B, b = pairs_dcsr(controls, rb, cbv, k2, n)
     ref = oracle.spgemm(A, B)
