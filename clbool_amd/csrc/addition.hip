@@ -19,27 +19,66 @@
 
 namespace clb {
 
-constexpr int MERGE_ITEMS = 8;
+// Merge of two ascending key lists, in tiles of MG_TILE outputs.  A first
+// pass finds where every tile boundary cuts A (merge_path.cl:38-90: the
+// number of A elements among the first d outputs, A first on ties), one
+// thread per boundary.  Each workgroup then loads its tile's pieces of A and B
+// into LDS with coalesced loads, every thread merges MG_ITEMS consecutive
+// outputs from LDS (its own merge-path search within the tile), and the tile
+// is written back with coalesced stores.
+constexpr int MG_WG = 256, MG_ITEMS = 8, MG_TILE = MG_WG * MG_ITEMS;
 
-__global__ __launch_bounds__(256) void k_merge_keys(const u64 *__restrict__ A, u64 na, const u64 *__restrict__ B,
-                                                    u64 nb, u64 *__restrict__ out) {
+__global__ void k_merge_partition(const u64 *__restrict__ A, u64 na, const u64 *__restrict__ B, u64 nb, u64 ntiles,
+                                  u64 *__restrict__ split) {
+    const u64 t = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t > ntiles) return;
     const u64 total = na + nb;
-    const u64 d0 = ((u64)blockIdx.x * 256 + threadIdx.x) * MERGE_ITEMS;
-    if (d0 >= total) return;
-    // merge_path.cl:38-90: elements of A among the first d0 outputs (A first on ties)
-    u64 lo = d0 > nb ? d0 - nb : 0, hi = d0 < na ? d0 : na;
+    const u64 d = t * MG_TILE < total ? t * MG_TILE : total;
+    u64 lo = d > nb ? d - nb : 0, hi = d < na ? d : na;
     while (lo < hi) {
         const u64 mid = (lo + hi) >> 1;
-        if (A[mid] <= B[d0 - 1 - mid]) lo = mid + 1;
+        if (A[mid] <= B[d - 1 - mid]) lo = mid + 1;
         else hi = mid;
     }
-    u64 i = lo, j = d0 - lo;
+    split[t] = lo;
+}
+
+__global__ __launch_bounds__(MG_WG) void k_merge_tiles(const u64 *__restrict__ A, u64 na, const u64 *__restrict__ B,
+                                                       u64 nb, const u64 *__restrict__ split,
+                                                       u64 *__restrict__ out) {
+    __shared__ u64 s[MG_TILE];
+    const u64 total = na + nb;
+    const u64 d0 = (u64)blockIdx.x * MG_TILE;
+    const u64 d1 = d0 + MG_TILE < total ? d0 + MG_TILE : total;
+    const u64 a0 = split[blockIdx.x], a1 = split[blockIdx.x + 1];
+    const u64 b0 = d0 - a0;
+    const u32 la = (u32)(a1 - a0), n = (u32)(d1 - d0), lb = n - la;
+    for (u32 t = threadIdx.x; t < n; t += MG_WG) s[t] = t < la ? A[a0 + t] : B[b0 + (t - la)];
+    __syncthreads();
+    const u32 k0 = threadIdx.x * MG_ITEMS;
+    u64 v[MG_ITEMS];
+    if (k0 < n) {
+        u32 lo = k0 > lb ? k0 - lb : 0, hi = k0 < la ? k0 : la;
+        while (lo < hi) {
+            const u32 mid = (lo + hi) >> 1;
+            if (s[mid] <= s[la + k0 - 1 - mid]) lo = mid + 1;
+            else hi = mid;
+        }
+        u32 i = lo, j = k0 - lo;
 #pragma unroll
-    for (int k = 0; k < MERGE_ITEMS; ++k) {
-        if (d0 + k >= total) break;
-        const bool take_a = i < na && (j >= nb || A[i] <= B[j]);
-        out[d0 + k] = take_a ? A[i++] : B[j++];
+        for (int k = 0; k < MG_ITEMS; ++k) {
+            const bool take_a = i < la && (j >= lb || s[i] <= s[la + j]);
+            v[k] = s[take_a ? i : (j < lb ? la + j : n - 1)];  // n - 1: past the tile, never stored
+            i += take_a;
+            j += !take_a;
+        }
     }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < MG_ITEMS; ++k)
+        if (k0 + k < n) s[k0 + k] = v[k];
+    __syncthreads();
+    for (u32 t = threadIdx.x; t < n; t += MG_WG) out[d0 + t] = s[t];
 }
 
 // Keys of a CSR matrix, one wave per row.
@@ -74,13 +113,22 @@ __global__ void k_rows_lower_bound(const u32 *__restrict__ rows, u32 m, u32 nrow
     rpt[r] = lo;
 }
 
+void merge_keys(clb_controls_s *ctl, const u64 *a, u64 na, const u64 *b, u64 nb, u64 *out) {
+    if (na + nb == 0) return;
+    const u64 tiles = (na + nb + MG_TILE - 1) / MG_TILE;
+    if (tiles >= 0xFFFFFFFFull) raise(INVALID_VALUE, "merge larger than the grid allows");
+    DevArray<u64> split(ctl, tiles + 1);
+    k_merge_partition<<<div_up(tiles + 1, 256), 256, 0, ctl->stream>>>(a, na, b, nb, tiles, split.ptr);
+    CLB_LAUNCHED("merge_partition");
+    k_merge_tiles<<<(u32)tiles, MG_WG, 0, ctl->stream>>>(a, na, b, nb, split.ptr, out);
+    CLB_LAUNCHED("merge");
+}
+
 // Merge two sorted key lists and drop duplicates -> (rows, cols) buffers.
 static u32 union_keys(clb_controls_s *ctl, const u64 *ka, u64 na, const u64 *kb, u64 nb, clb_buffer_s **rows,
                       clb_buffer_s **cols) {
     DevArray<u64> merged(ctl, na + nb);
-    const u64 threads = (na + nb + MERGE_ITEMS - 1) / MERGE_ITEMS;
-    k_merge_keys<<<div_up(threads, 256), 256, 0, ctl->stream>>>(ka, na, kb, nb, merged.ptr);
-    CLB_LAUNCHED("merge");
+    merge_keys(ctl, ka, na, kb, nb, merged.ptr);
     return unique_keys_to_pairs(ctl, merged.ptr, na + nb, rows, cols);
 }
 
@@ -228,17 +276,9 @@ clb_status clb_dcsr_addition(clb_controls ctl, const clb_dcsr *a, const clb_dcsr
         k_dcsr_keys<<<div_up(b->nzr, 4), 256, 0, ctl->stream>>>(u32p(b->rpt), u32p(b->rows), u32p(b->cols), b->nzr,
                                                                  kb.ptr);
         CLB_LAUNCHED("dcsr_keys");
-        clb_buffer_s *rows = nullptr;
-        const u32 m = union_keys(ctl, ka.ptr, a->nnz, kb.ptr, b->nnz, &rows, &r.cols);
-        try {
-            rows_to_dcsr(ctl, u32p(rows), m, &r.rpt, &r.rows, &r.nzr);
-        } catch (...) {
-            buffer_release(rows);
-            clb_dcsr_release(&r);
-            throw;
-        }
-        buffer_release(rows);
-        r.nnz = m;
+        DevArray<u64> merged(ctl, (u64)a->nnz + b->nnz);
+        merge_keys(ctl, ka.ptr, a->nnz, kb.ptr, b->nnz, merged.ptr);
+        keys_to_dcsr(ctl, merged.ptr, (u64)a->nnz + b->nnz, r);
         *out = r;
     });
 }

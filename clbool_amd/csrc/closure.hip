@@ -11,8 +11,10 @@
 // rows of all products.  The product is taken in row blocks of Y' whose
 // output bound (sum over rows of min(products, n)) stays below 2^31, so each
 // block's result fits uint32 indices however large the step grows.  Each X's
-// union is then one sort and deduplication of its own keys and of the product
-// blocks that target it.  Every step of both modes is bit-exact: the products
+// union merges sorted key segments -- its own keys and, per triple that
+// targets it, that triple's product rows (already sorted: a product block is
+// canonical DCSR) -- and drops duplicates.  Every step of both modes is
+// bit-exact: the products
 // and unions are exact set operations, only the launch structure differs.
 #include "sort.hpp"
 #include "spgemm.hpp"
@@ -74,16 +76,6 @@ __global__ __launch_bounds__(256) void k_keys_shifted(const u32 *__restrict__ rp
     const u64 hi = (u64)(rows[r] + row_shift) << 32;
     const u32 e = rpt[r + 1];
     for (u32 j = rpt[r] + dev::lane(); j < e; j += 64) keys[j] = hi | cols[j];
-}
-
-// Sorted unique keys [k0, k0 + cnt) -> (rows - row_shift, cols) arrays.
-__global__ void k_unstack_pairs(const u64 *__restrict__ keys, u64 k0, u32 cnt, u32 row_shift, u32 *__restrict__ rows,
-                                u32 *__restrict__ cols) {
-    const u32 e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= cnt) return;
-    const u64 k = keys[k0 + e];
-    rows[e] = (u32)(k >> 32) - row_shift;
-    cols[e] = (u32)k;
 }
 
 // Keys (row << 32 | col) of stored rows [r0, r1) of a DCSR matrix, rows
@@ -312,8 +304,11 @@ clb_status clb_closure_step(clb_controls ctl, clb_dcsr *mats, uint32_t nmats, co
             clb_dcsr_release(&Y);
             clb_dcsr_release(&Z);
             try {
-                // one union per distinct X: its keys and every product block
-                // that targets it, sorted and deduplicated
+                // one union per distinct X.  Its keys and, per triple that
+                // targets it, the keys of that triple's product rows (its
+                // parts in consecutive row blocks concatenate in row order)
+                // form sorted segments; rounds of pairwise merges turn them
+                // into one sorted list, whose duplicates are then dropped.
                 for (u32 k = 0; k < (u32)xs.size(); ++k) {
                     clb_dcsr &x = mats[xs[k]];
                     u64 seg = x.nnz;
@@ -323,19 +318,22 @@ clb_status clb_closure_step(clb_controls ctl, clb_dcsr *mats, uint32_t nmats, co
                     if (seg == x.nnz) continue;
                     if (seg >= 0xFFFFFFFFull)
                         raise(INVALID_VALUE, "a closure union exceeds uint32 nonzeros");
-                    DevArray<u64> keys(ctl, seg);
+                    DevArray<u64> keys(ctl, seg), tmp(ctl, seg);
+                    std::vector<std::pair<u64, u64>> segs;  // (start, length) in keys
                     u64 pos = 0;
                     if (x.nnz) {
                         k_keys_shifted<<<div_up(x.nzr, 4), 256, 0, ctl->stream>>>(u32p(x.rpt), u32p(x.rows),
                                                                                    u32p(x.cols), x.nzr, 0, keys.ptr);
                         CLB_LAUNCHED("keys");
+                        segs.push_back({0, x.nnz});
                         pos = x.nnz;
                     }
-                    for (size_t q = 0; q < prods.size(); ++q) {
-                        const clb_dcsr &T = prods[q];
-                        const std::vector<u32> &hs = splits[q];
-                        for (u32 t = 0; t < ntriples; ++t) {
-                            if (triples[3 * t] != xs[k]) continue;
+                    for (u32 t = 0; t < ntriples; ++t) {
+                        if (triples[3 * t] != xs[k]) continue;
+                        const u64 start = pos;
+                        for (size_t q = 0; q < prods.size(); ++q) {
+                            const clb_dcsr &T = prods[q];
+                            const std::vector<u32> &hs = splits[q];
                             const u32 r0 = hs[2 * t], r1 = hs[2 * t + 2];
                             const u32 e0 = hs[2 * t + 1], e1 = hs[2 * t + 3];
                             if (e1 == e0) continue;
@@ -346,26 +344,35 @@ clb_status clb_closure_step(clb_controls ctl, clb_dcsr *mats, uint32_t nmats, co
                             CLB_LAUNCHED("keys_range");
                             pos += e1 - e0;
                         }
+                        if (pos > start) segs.push_back({start, pos - start});
                     }
-                    sort_pair_keys(ctl, keys.ptr, pos, n - 1, n - 1);
-                    DevArray<u64> uk;
-                    const u32 m = unique_keys(ctl, keys.ptr, pos, uk);
-                    if (m == x.nnz) continue;  // unions only add: equal size means unchanged
+                    u64 *cur = keys.ptr, *nxt = tmp.ptr;
+                    while (segs.size() > 1) {
+                        std::vector<std::pair<u64, u64>> next;
+                        for (size_t i = 0; i < segs.size(); i += 2) {
+                            const auto &sa = segs[i];
+                            if (i + 1 == segs.size()) {  // odd one out: copied to the other buffer
+                                CLB_HIP(hipMemcpyAsync(nxt + sa.first, cur + sa.first, 8 * sa.second,
+                                                       hipMemcpyDeviceToDevice, ctl->stream),
+                                        COPY_BUFFER_ERROR);
+                                next.push_back(sa);
+                                continue;
+                            }
+                            const auto &sb = segs[i + 1];  // adjacent: sb.first == sa.first + sa.second
+                            merge_keys(ctl, cur + sa.first, sa.second, cur + sb.first, sb.second, nxt + sa.first);
+                            next.push_back({sa.first, sa.second + sb.second});
+                        }
+                        segs.swap(next);
+                        std::swap(cur, nxt);
+                    }
                     clb_dcsr nx{};
                     nx.nrows = n;
                     nx.ncols = n;
-                    DevArray<u32> rr(ctl, m);
-                    nx.cols = buffer_new(ctl, 4ull * m);
-                    k_unstack_pairs<<<div_up(m, 256), 256, 0, ctl->stream>>>(uk.ptr, 0, m, 0, rr.ptr,
-                                                                              u32p(nx.cols));
-                    CLB_LAUNCHED("unstack");
-                    try {
-                        rows_to_dcsr(ctl, rr.ptr, m, &nx.rpt, &nx.rows, &nx.nzr);
-                    } catch (...) {
+                    keys_to_dcsr(ctl, cur, pos, nx);
+                    if (nx.nnz == x.nnz) {  // unions only add: equal size means unchanged
                         clb_dcsr_release(&nx);
-                        throw;
+                        continue;
                     }
-                    nx.nnz = m;
                     clb_dcsr_release(&x);
                     x = nx;
                 }

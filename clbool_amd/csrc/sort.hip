@@ -137,43 +137,150 @@ __global__ void k_key_max(const u64 *__restrict__ keys, u64 n, u32 *__restrict__
 }
 
 // flag[i] = key[i] != key[i-1] (first always kept), flag[n] = 0
-__global__ void k_unique_flags(const u64 *__restrict__ keys, u64 n, u32 *__restrict__ flags) {
-    const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) flags[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1u : 0u;
-    if (i == n) flags[n] = 0;
+// Sorted keys -> distinct keys, in tiles of KT_TILE keys.  Key i is kept
+// when it differs from key i - 1, and starts a row among the kept keys when
+// its row also differs (the previous distinct key is key i - 1).  A count
+// pass gives each tile's kept keys and row starts, their scans give each
+// tile's output offsets, and the scatter pass places every kept key by its
+// rank inside the tile (ballots per wave and item, counts exchanged in LDS).
+// Loads are striped (item it of thread t is key it * KT_WG + t of the tile),
+// so every load and every run of stores is coalesced.
+constexpr int KT_WG = 256, KT_ITEMS = 16;
+constexpr u64 KT_TILE = (u64)KT_WG * KT_ITEMS;
+
+// bit 0: kept, bit 1: starts a row among the kept keys
+__device__ __forceinline__ u32 key_flags(const u64 *__restrict__ keys, u64 i, u64 k) {
+    if (i == 0) return 3u;
+    const u64 p = keys[i - 1];
+    return k == p ? 0u : ((k >> 32) != (p >> 32) ? 3u : 1u);
 }
 
-__global__ void k_unique_scatter(const u64 *__restrict__ keys, u64 n, const u32 *__restrict__ pos,
-                                 u32 *__restrict__ rows, u32 *__restrict__ cols) {
-    const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n && pos[i] != pos[i + 1]) {
-        rows[pos[i]] = (u32)(keys[i] >> 32);
-        cols[pos[i]] = (u32)keys[i];
+__global__ __launch_bounds__(KT_WG) void k_keys_count(const u64 *__restrict__ keys, u64 n, u32 *__restrict__ kept,
+                                                      u32 *__restrict__ starts) {
+    __shared__ u32 red[KT_WG / dev::WAVE + 1];
+    const u64 base = (u64)blockIdx.x * KT_TILE;
+    u32 cu = 0, cs = 0;
+#pragma unroll 4
+    for (int it = 0; it < KT_ITEMS; ++it) {
+        const u64 i = base + (u64)it * KT_WG + threadIdx.x;
+        if (i < n) {
+            const u32 f = key_flags(keys, i, keys[i]);
+            cu += f & 1u;
+            cs += f >> 1;
+        }
+    }
+    const u32 tu = dev::block_sum<u32, KT_WG>(cu, red);
+    const u32 ts = dev::block_sum<u32, KT_WG>(cs, red);
+    if (threadIdx.x == 0) {
+        kept[blockIdx.x] = tu;
+        starts[blockIdx.x] = ts;
     }
 }
 
-__global__ void k_unique_key_scatter(const u64 *__restrict__ keys, u64 n, const u32 *__restrict__ pos,
-                                     u64 *__restrict__ out) {
-    const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n && pos[i] != pos[i + 1]) out[pos[i]] = keys[i];
+// DCSR: cols[p] = column of the p-th kept key; for the q-th row start,
+// rows[q] = its row and rpt[q] = p; rpt[nzr] = m.  Pairs (!DCSR): rows[p] =
+// row of the p-th kept key.
+template <bool DCSR>
+__global__ __launch_bounds__(KT_WG) void k_keys_scatter(const u64 *__restrict__ keys, u64 n,
+                                                        const u32 *__restrict__ koff, const u32 *__restrict__ soff,
+                                                        const u32 *__restrict__ totals, u32 *__restrict__ cols,
+                                                        u32 *__restrict__ rows, u32 *__restrict__ rpt) {
+    constexpr int NW = KT_WG / dev::WAVE;
+    __shared__ u32 cu[KT_ITEMS][NW], cs[KT_ITEMS][NW];
+    const u64 base = (u64)blockIdx.x * KT_TILE;
+    const u32 w = dev::wave_id();
+    const u32 l = (u32)dev::lane();
+    u64 k[KT_ITEMS];
+    u32 fl = 0;  // bit it: kept, bit 16 + it: row start
+#pragma unroll
+    for (int it = 0; it < KT_ITEMS; ++it) {
+        const u64 i = base + (u64)it * KT_WG + threadIdx.x;
+        k[it] = i < n ? keys[i] : 0ull;
+        const u32 f = i < n ? key_flags(keys, i, k[it]) : 0u;
+        fl |= (f & 1u) << it;
+        fl |= (f >> 1) << (16 + it);
+        const u64 bu = __ballot(f & 1u), bs = __ballot(f >> 1);
+        if (l == 0) {
+            cu[it][w] = (u32)__popcll(bu);
+            cs[it][w] = (u32)__popcll(bs);
+        }
+    }
+    __syncthreads();
+    u32 au = koff[blockIdx.x], as = soff[blockIdx.x];
+#pragma unroll
+    for (int it = 0; it < KT_ITEMS; ++it) {
+        u32 lu = 0, ls = 0, tu = 0, ts = 0;
+#pragma unroll
+        for (int v = 0; v < NW; ++v) {
+            const u32 x = cu[it][v], y = cs[it][v];
+            tu += x;
+            ts += y;
+            lu += (u32)v < w ? x : 0u;
+            ls += (u32)v < w ? y : 0u;
+        }
+        const bool kept = (fl >> it) & 1u, start = (fl >> (16 + it)) & 1u;
+        const u64 bu = __ballot(kept), bs = __ballot(start);
+        if (kept) {
+            const u32 p = au + lu + dev::lanes_below(bu);
+            cols[p] = (u32)k[it];
+            if (!DCSR) rows[p] = (u32)(k[it] >> 32);
+            if (DCSR && start) {
+                const u32 q = as + ls + dev::lanes_below(bs);
+                rows[q] = (u32)(k[it] >> 32);
+                rpt[q] = p;
+            }
+        }
+        au += tu;
+        as += ts;
+    }
+    if (DCSR && blockIdx.x == 0 && threadIdx.x == 0) rpt[totals[1]] = totals[0];
 }
 
-u32 unique_keys(clb_controls_s *ctl, const u64 *keys, u64 n, DevArray<u64> &out) {
+// Counts and offsets of the kept keys and row starts: tot[0] = kept keys,
+// tot[1] = row starts.
+static void keys_offsets(clb_controls_s *ctl, const u64 *keys, u64 n, DevArray<u32> &koff, DevArray<u32> &soff,
+                         DevArray<u32> &tot, u32 &m, u32 &nzr) {
+    if (n >= 0xFFFFFFFFull) raise(INVALID_VALUE, "more keys than uint32 indices allow");
+    const u64 tiles = (n + KT_TILE - 1) / KT_TILE;
+    koff.reset(ctl, tiles);
+    soff.reset(ctl, tiles);
+    tot.reset(ctl, 2);
+    k_keys_count<<<(u32)tiles, KT_WG, 0, ctl->stream>>>(keys, n, koff.ptr, soff.ptr);
+    CLB_LAUNCHED("keys_count");
+    exclusive_scan_u32(ctl, koff.ptr, koff.ptr, tiles, tot.ptr);
+    exclusive_scan_u32(ctl, soff.ptr, soff.ptr, tiles, tot.ptr + 1);
+    u32 h[2];
+    readback(ctl, h, tot.ptr, 8);
+    m = h[0];
+    nzr = h[1];
+}
+
+void keys_to_dcsr(clb_controls_s *ctl, const u64 *keys, u64 n, clb_dcsr &out) {
+    clb_dcsr r{};
+    r.nrows = out.nrows;
+    r.ncols = out.ncols;
     if (n == 0) {
-        out.reset(ctl, 1);
-        return 0;
+        out = r;
+        return;
     }
-    DevArray<u32> pos(ctl, n + 1);
-    k_unique_flags<<<div_up(n + 1, 256), 256, 0, ctl->stream>>>(keys, n, pos.ptr);
-    CLB_LAUNCHED("unique_flags");
-    DevArray<u32> tot(ctl, 1);
-    exclusive_scan_u32(ctl, pos.ptr, pos.ptr, n + 1, tot.ptr);
-    u32 m = 0;
-    readback(ctl, &m, tot.ptr, 4);
-    out.reset(ctl, m ? m : 1);
-    k_unique_key_scatter<<<div_up(n, 256), 256, 0, ctl->stream>>>(keys, n, pos.ptr, out.ptr);
-    CLB_LAUNCHED("unique_key_scatter");
-    return m;
+    DevArray<u32> koff, soff, tot;
+    u32 m = 0, nzr = 0;
+    keys_offsets(ctl, keys, n, koff, soff, tot, m, nzr);
+    try {
+        r.cols = buffer_new(ctl, 4ull * m);
+        r.rows = buffer_new(ctl, 4ull * nzr);
+        r.rpt = buffer_new(ctl, 4ull * (nzr + 1ull));
+        k_keys_scatter<true><<<(u32)koff.n, KT_WG, 0, ctl->stream>>>(keys, n, koff.ptr, soff.ptr, tot.ptr,
+                                                                      u32p(r.cols), u32p(r.rows), u32p(r.rpt));
+        CLB_LAUNCHED("keys_scatter");
+        sync(ctl);
+    } catch (...) {
+        clb_dcsr_release(&r);
+        throw;
+    }
+    r.nnz = m;
+    r.nzr = nzr;
+    out = r;
 }
 
 void pack_pairs(clb_controls_s *ctl, const u32 *rows, const u32 *cols, u64 n, u64 *keys) {
@@ -191,17 +298,26 @@ void unpack_pairs(clb_controls_s *ctl, const u64 *keys, u64 n, u32 *rows, u32 *c
 // Sorted keys -> deduplicated (rows, cols) buffers.
 u32 unique_keys_to_pairs(clb_controls_s *ctl, const u64 *keys, u64 n, clb_buffer_s **rows_out,
                          clb_buffer_s **cols_out) {
-    DevArray<u32> pos(ctl, n + 1);
-    k_unique_flags<<<div_up(n + 1, 256), 256, 0, ctl->stream>>>(keys, n, pos.ptr);
-    CLB_LAUNCHED("unique_flags");
-    DevArray<u32> tot(ctl, 1);
-    exclusive_scan_u32(ctl, pos.ptr, pos.ptr, n + 1, tot.ptr);
-    u32 m = 0;
-    readback(ctl, &m, tot.ptr, 4);
+    *rows_out = *cols_out = nullptr;
+    if (n == 0) {
+        *rows_out = buffer_new(ctl, 4);
+        *cols_out = buffer_new(ctl, 4);
+        return 0;
+    }
+    DevArray<u32> koff, soff, tot;
+    u32 m = 0, nzr = 0;
+    keys_offsets(ctl, keys, n, koff, soff, tot, m, nzr);
     *rows_out = buffer_new(ctl, 4ull * (m ? m : 1));
-    *cols_out = buffer_new(ctl, 4ull * (m ? m : 1));
-    k_unique_scatter<<<div_up(n, 256), 256, 0, ctl->stream>>>(keys, n, pos.ptr, u32p(*rows_out), u32p(*cols_out));
-    CLB_LAUNCHED("unique_scatter");
+    try {
+        *cols_out = buffer_new(ctl, 4ull * (m ? m : 1));
+    } catch (...) {
+        buffer_release(*rows_out);
+        *rows_out = nullptr;
+        throw;
+    }
+    k_keys_scatter<false><<<(u32)koff.n, KT_WG, 0, ctl->stream>>>(keys, n, koff.ptr, soff.ptr, tot.ptr,
+                                                                   u32p(*cols_out), u32p(*rows_out), nullptr);
+    CLB_LAUNCHED("keys_scatter");
     return m;
 }
 

@@ -16,8 +16,12 @@ void key_bounds(clb_controls_s *ctl, const u64 *keys, u64 n, u32 *max_row, u32 *
 // Sorted keys -> new deduplicated (rows, cols) buffers; returns their length.
 u32 unique_keys_to_pairs(clb_controls_s *ctl, const u64 *keys, u64 n, clb_buffer_s **rows_out,
                          clb_buffer_s **cols_out);
-// Sorted keys -> new array of the distinct keys; returns how many.
-u32 unique_keys(clb_controls_s *ctl, const u64 *keys, u64 n, DevArray<u64> &out);
+// Merge of two ascending key lists into out[0, na + nb) (merge path, kept
+// duplicates).
+void merge_keys(clb_controls_s *ctl, const u64 *a, u64 na, const u64 *b, u64 nb, u64 *out);
+// Sorted keys (row << 32 | col, duplicates allowed) -> a new canonical DCSR
+// of their distinct pairs; out.nrows/ncols are kept as given.
+void keys_to_dcsr(clb_controls_s *ctl, const u64 *keys, u64 n, clb_dcsr &out);
 void coo_finish(clb_controls_s *ctl, clb_buffer_s *rows, clb_buffer_s *cols, u32 nrows, u32 ncols, u32 nnz,
                 bool sorted, bool nodup, clb_coo &out);
 

@@ -131,36 +131,49 @@ __global__ void k_set_positions(const u32 *__restrict__ nnz_row, const u64 *__re
 }
 
 // Expands one dumped bitmap row (one workgroup per row) into its columns at
-// their final positions.  Each of the 16 waves owns a contiguous sixteenth of
+// their final positions.  Each of the 4 waves owns a contiguous quarter of
 // the row's words; its start position is the prefix of the chunk counts
 // before its first chunk, so the waves need no exchange.  A wave loads its
-// words 8 steps at a time and writes them through its own LDS staging area
-// (dev::emit_step_staged), so every store instruction is one contiguous run
-// of columns.
+// words 8 steps at a time, double-buffered (the next 8 steps' loads are in
+// flight while the current ones are expanded), and writes them through its
+// own LDS staging area (dev::emit_step_staged), so every store instruction
+// is one contiguous run of columns.  Small workgroups let four rows share a
+// CU, so one row's load latency overlaps another's expansion.
 // gpre = exclusive prefix of chunk_cnt over all slots.
-constexpr int EMIT_WG = 1024;
+constexpr int EMIT_WG = 256;
 __global__ __launch_bounds__(EMIT_WG) void k_bitmap_emit(const uint4 *__restrict__ store, u32 nch,
                                                          const u64 *__restrict__ gpre, const u32 *__restrict__ list,
                                                          const u32 *__restrict__ perm, const u64 *__restrict__ off,
-                                                         u32 words_per_row, u32 dense, u32 *__restrict__ cols) {
-    constexpr u32 NW = EMIT_WG / 64, BATCH = 8;
+                                                         u32 words_per_row, u32 *__restrict__ cols) {
+    constexpr u32 NW = EMIT_WG / 64, BATCH = 8, SPAN = BATCH * 64;
     __shared__ u32 stage[NW][dev::STAGE_WORDS];
     const u32 slot = list[blockIdx.x];
     const u32 r = perm[slot];
     u32 *out = cols + off[r];
     const u64 lim = off[r + 1] - off[r];
     const u32 wv = dev::wave_id(), l = (u32)dev::lane();
-    const u32 wpw = words_per_row / NW;  // a multiple of EMIT_CHUNK: the window is >= 2^14 words
+    // words_per_row is a multiple of 2^14, so wpw is a multiple of 2 * SPAN
+    // and of EMIT_CHUNK
+    const u32 wpw = words_per_row / NW;
     const u32 first = wv * wpw;
     const u32 *words = reinterpret_cast<const u32 *>(store) + (u64)slot * words_per_row + first;
     u64 p = gpre[(u64)slot * nch + first / EMIT_CHUNK] - gpre[(u64)slot * nch];
-    for (u32 s0 = 0; s0 < wpw; s0 += BATCH * 64) {
-        u32 w[BATCH];
+    auto load = [&](u32(&w)[BATCH], u32 s0) {
 #pragma unroll
         for (u32 k = 0; k < BATCH; ++k) w[k] = words[s0 + k * 64 + l];
+    };
+    auto expand = [&](const u32(&w)[BATCH], u32 s0) {
 #pragma unroll
         for (u32 k = 0; k < BATCH; ++k)
             p += dev::emit_step_staged(w[k], (first + s0 + k * 64) * 32u, p, out, lim, stage[wv]);
+    };
+    u32 wa[BATCH], wb[BATCH];
+    load(wa, 0);
+    for (u32 s0 = 0; s0 < wpw; s0 += 2 * SPAN) {
+        load(wb, s0 + SPAN);
+        expand(wa, s0);
+        if (s0 + 2 * SPAN < wpw) load(wa, s0 + 2 * SPAN);
+        expand(wb, s0 + SPAN);
     }
 }
 
@@ -205,10 +218,10 @@ void emit_dense(clb_controls_s *ctl, const u32 *perm, const BinHost &h, DumpBufs
     if (!count) return;
     DevArray<u64> gpre(ctl, d.cnt.n + 1);
     exclusive_scan_u32_to_u64(ctl, d.cnt.ptr, gpre.ptr, d.cnt.n, gpre.ptr + d.cnt.n);
-    const u32 dense = ctl->tuning.emit_lane_bits_min ? ctl->tuning.emit_lane_bits_min : EMIT_DEFAULT;
+    if (d.words % (1u << 14)) raise(RUN_KERNEL_FAILURE, "dumped rows must hold whole 2^14-word windows");
     if (lt) lt->begin(ctl->stream);
     k_bitmap_emit<<<count, EMIT_WG, 0, ctl->stream>>>(d.store.ptr, d.nch, gpre.ptr, d.list.ptr,
-                                                       perm + h.off[BIN_BITMAP0], off, d.words, dense, cols);
+                                                       perm + h.off[BIN_BITMAP0], off, d.words, cols);
     CLB_LAUNCHED("bitmap_emit");
     if (lt) lt->end((int)CLB_KF(pass, CLB_KK_EMIT), ctl->stream);
     if (st) {
