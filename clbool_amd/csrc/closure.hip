@@ -98,14 +98,17 @@ __global__ void k_row_lengths(const u32 *__restrict__ rpt, const u32 *__restrict
 }
 
 // bound[r] = min(products of stored row r of Y', n): an upper bound of the
-// row's output size.
-__global__ void k_row_bounds(const u32 *__restrict__ rpt, const u32 *__restrict__ cols, u32 nzr,
-                             const u32 *__restrict__ zlen, u32 n, u64 *__restrict__ bound) {
-    const u32 r = blockIdx.x * blockDim.x + threadIdx.x;
+// row's output size.  One wave per row (rows of Y' reach thousands of
+// nonzeros).
+__global__ __launch_bounds__(256) void k_row_bounds(const u32 *__restrict__ rpt, const u32 *__restrict__ cols,
+                                                    u32 nzr, const u32 *__restrict__ zlen, u32 n,
+                                                    u64 *__restrict__ bound) {
+    const u32 r = blockIdx.x * 4 + dev::wave_id();
     if (r >= nzr) return;
     u64 p = 0;
-    for (u32 j = rpt[r]; j < rpt[r + 1]; ++j) p += zlen[cols[j]];
-    bound[r] = p < n ? p : n;
+    for (u32 j = rpt[r] + dev::lane(); j < rpt[r + 1]; j += dev::WAVE) p += zlen[cols[j]];
+    p = dev::wave_sum(p);
+    if (dev::lane() == 0) bound[r] = p < n ? p : n;
 }
 
 // cuts[b] = first stored row whose exclusive bound prefix is >= b * lim.
@@ -260,7 +263,7 @@ clb_status clb_closure_step(clb_controls ctl, clb_dcsr *mats, uint32_t nmats, co
                                                                                 zlen.ptr);
                     CLB_LAUNCHED("row_lengths");
                     DevArray<u64> pre(ctl, Y.nzr), tot(ctl, 1);
-                    k_row_bounds<<<div_up(Y.nzr, 256), 256, 0, ctl->stream>>>(u32p(Y.rpt), u32p(Y.cols), Y.nzr,
+                    k_row_bounds<<<div_up(Y.nzr, 4), 256, 0, ctl->stream>>>(u32p(Y.rpt), u32p(Y.cols), Y.nzr,
                                                                                zlen.ptr, n, pre.ptr);
                     CLB_LAUNCHED("row_bounds");
                     exclusive_scan_u64(ctl, pre.ptr, pre.ptr, Y.nzr, tot.ptr);

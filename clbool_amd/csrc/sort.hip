@@ -188,15 +188,16 @@ __global__ __launch_bounds__(KT_WG) void k_keys_scatter(const u64 *__restrict__ 
     constexpr int NW = KT_WG / dev::WAVE;
     __shared__ u32 cu[KT_ITEMS][NW], cs[KT_ITEMS][NW];
     const u64 base = (u64)blockIdx.x * KT_TILE;
+    const u64 *kt = keys + base;  // the tile, indexed by 32-bit offsets
+    const u32 nt = n - base < KT_TILE ? (u32)(n - base) : (u32)KT_TILE;
     const u32 w = dev::wave_id();
     const u32 l = (u32)dev::lane();
-    u64 k[KT_ITEMS];
     u32 fl = 0;  // bit it: kept, bit 16 + it: row start
-#pragma unroll
+    // pass 1: flags; per wave and item the counts go to LDS
+#pragma unroll 4
     for (int it = 0; it < KT_ITEMS; ++it) {
-        const u64 i = base + (u64)it * KT_WG + threadIdx.x;
-        k[it] = i < n ? keys[i] : 0ull;
-        const u32 f = i < n ? key_flags(keys, i, k[it]) : 0u;
+        const u32 o = (u32)it * KT_WG + threadIdx.x;
+        const u32 f = o < nt ? key_flags(keys, base + o, kt[o]) : 0u;
         fl |= (f & 1u) << it;
         fl |= (f >> 1) << (16 + it);
         const u64 bu = __ballot(f & 1u), bs = __ballot(f >> 1);
@@ -206,8 +207,9 @@ __global__ __launch_bounds__(KT_WG) void k_keys_scatter(const u64 *__restrict__ 
         }
     }
     __syncthreads();
+    // pass 2: ranks; the kept keys are read again (from cache)
     u32 au = koff[blockIdx.x], as = soff[blockIdx.x];
-#pragma unroll
+#pragma unroll 4
     for (int it = 0; it < KT_ITEMS; ++it) {
         u32 lu = 0, ls = 0, tu = 0, ts = 0;
 #pragma unroll
@@ -221,12 +223,13 @@ __global__ __launch_bounds__(KT_WG) void k_keys_scatter(const u64 *__restrict__ 
         const bool kept = (fl >> it) & 1u, start = (fl >> (16 + it)) & 1u;
         const u64 bu = __ballot(kept), bs = __ballot(start);
         if (kept) {
+            const u64 k = kt[(u32)it * KT_WG + threadIdx.x];
             const u32 p = au + lu + dev::lanes_below(bu);
-            cols[p] = (u32)k[it];
-            if (!DCSR) rows[p] = (u32)(k[it] >> 32);
+            cols[p] = (u32)k;
+            if (!DCSR) rows[p] = (u32)(k >> 32);
             if (DCSR && start) {
                 const u32 q = as + ls + dev::lanes_below(bs);
-                rows[q] = (u32)(k[it] >> 32);
+                rows[q] = (u32)(k >> 32);
                 rpt[q] = p;
             }
         }
