@@ -243,15 +243,17 @@ BinCfg make_bincfg(const clb_controls_s *ctl, u32 ncols) {
     c.bitmap_ratio = ctl->tuning.bitmap_slot_ratio;
     // Window of a bitmap kernel: the matrix width rounded up to a power of
     // two words, at most 2^15 words (128 KB of LDS, one 1024-thread workgroup
-    // per CU) for the symbolic count and at most 2^14 (64 KB, two 512-thread
-    // workgroups per CU) for the kernels that also write columns: their
-    // emission phase then overlaps the other workgroup's gather.  Wider
-    // matrices are processed in column slices.
+    // per CU); wider matrices are processed in column slices.  The kernels
+    // that also write columns use 2^14 words (64 KB, two 512-thread
+    // workgroups per CU, so one row's emission overlaps the other's gather)
+    // when that costs at most one extra slice, i.e. up to 2^20 columns; on
+    // wider matrices the extra slices (each re-reads the A row and its B-row
+    // pieces) cost more than the overlap gains.
     const u64 words = ((u64)ncols + 31) / 32;
     u32 wb = 6;
     while (wb < 15 && (1ull << wb) < words) ++wb;
     c.sym_bits = wb;
-    c.bm_bits = wb < 14 ? wb : 14;
+    c.bm_bits = (wb == 15 && words <= (1ull << 15)) ? 14 : wb;
     return c;
 }
 
@@ -501,7 +503,7 @@ void launch_bins(clb_controls_s *ctl, const Setup &s, const u32 *perm, const u32
                 case 12: BITMAP3(12, 512); break;
                 case 13: BITMAP3(13, 512); break;
                 case 14: BITMAP3(14, 512); break;
-                case 15: launch_bitmap<15, 1024, SYM>(ctl, st, s, pp, n, o); break;  // symbolic only
+                case 15: BITMAP3(15, 1024); break;
             }
 #undef BITMAP3
             CLB_LAUNCHED("bitmap");

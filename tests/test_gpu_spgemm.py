@@ -224,9 +224,8 @@ def test_dense_bitmap_rows(controls, dense_knob, persistent):
 @pytest.mark.parametrize("ncols", [1 << 22, (1 << 20) + 12345])
 @pytest.mark.parametrize("dense_knob", [4, 64, 65536])
 def test_dense_rows_over_column_slices(controls, dense_knob, ncols):
-    """Columns over several bitmap windows (2^22: four symbolic windows of
-    2^20 columns and eight numeric ones of 2^19; 2^20 + 12345: a partial last
-    window, two symbolic and three numeric): rows whose first slice is dense
+    """Columns over several bitmap windows of 2^20 columns (2^22: four;
+    2^20 + 12345: two, the last one partial): rows whose first slice is dense
     are dumped slice by slice and expanded by the emission kernel; the others
     are emitted slice by slice in LDS.  Both passes and the learned pass must
     agree with the oracle."""
