@@ -193,6 +193,105 @@ def bench_closure(cb, gen, ctl, max_iters, log, seed=0):
     return out
 
 
+def _dcsr_rows(rpt, rows, cols, keep):
+    """Host DCSR restricted to the stored rows whose positions are in `keep`
+    (ascending); row ids unchanged."""
+    rpt = rpt.astype(np.int64)
+    lens = rpt[keep + 1] - rpt[keep]
+    starts = np.repeat(rpt[keep] - np.r_[0, np.cumsum(lens)[:-1]], lens)
+    idx = starts + np.arange(int(lens.sum()), dtype=np.int64)
+    return np.r_[0, np.cumsum(lens)].astype(np.uint32), rows[keep].astype(np.uint32), cols[idx].astype(np.uint32)
+
+
+def bench_c5(cb, gen, ctl, log, scale=22, ef=32, step_scale=16, seed=1):
+    """The RPQ input of a 64x64 automaton M (200 uniform nonzeros) and an
+    R-MAT graph G (scale 22, edge factor 32).  K = M (x) G has about 2.6e10
+    nonzeros, more than one uint32 DCSR holds, so it is built in blocks of
+    automaton rows (K rows i * 2^scale + g) and timed as a write-bound kernel.
+    One closure step K * K at this scale is products(M*M) * products(G*G),
+    about 3e14 products, beyond one GPU; the step itself is measured on the
+    same automaton with G at scale `step_scale` (K of width 64 * 2^16 = 2^22),
+    whole, in row blocks of at most 2^31 products, plain and learned; both
+    results must agree."""
+    mr, mc = gen.uniform_pairs(200, 64, 64, seed)
+    mrpt, mrows, mcols = gen.to_dcsr(mr, mc)
+    out = {"automaton": {"n": 64, "nnz": int(mr.size), "seed": seed}}
+    t0 = time.perf_counter()
+    r, c = gen.rmat(scale, ef, seed)
+    grpt, grows, gcols = gen.to_dcsr(r, c)
+    del r, c
+    n = 1 << scale
+    log(f"[bench] C5: G nnz={gcols.size} generated in {time.perf_counter() - t0:.1f} s")
+    G = cb.matrix_dcsr_from_cpu(ctl, grpt, grows, gcols, n, n)
+    g_products = int(row_products(grpt, grows, gcols, n, grpt, grows).sum())
+    out["graph"] = {"scale": scale, "edge_factor": ef, "seed": seed, "nnz": int(gcols.size), "nzr": int(grows.size),
+                    "products_g_times_g": g_products}
+    del grpt, grows, gcols
+    # Kronecker, one automaton row at a time (each block < 2^32 nonzeros)
+    kron_nnz, kron_bytes, kron_s = 0, 0, 0.0
+    for q in range(mrows.size):
+        M1 = cb.matrix_dcsr_from_cpu(ctl, *_dcsr_rows(mrpt, mrows, mcols, np.array([q])), 64, 64)
+        ctl.sync()
+        t = time.perf_counter()
+        K = cb.dcsr.kronecker_product(ctl, M1, G)
+        ctl.sync()
+        kron_s += time.perf_counter() - t
+        kron_nnz += K.nnz
+        kron_bytes += 4 * K.nnz + 8 * K.nzr + 4
+        del K
+    del G
+    out["kronecker"] = {"blocks": int(mrows.size), "nnz": kron_nnz, "bytes_written": kron_bytes,
+                        "seconds": kron_s, "GB_per_s": kron_bytes / kron_s / 1e9,
+                        "frac_of_hbm_peak": kron_bytes / kron_s / 1e9 / HBM_PEAK_GBS,
+                        "note": "host-timed per block (includes the size readback and pooled allocation)"}
+    log(f"[bench] C5 Kronecker: {kron_nnz} nnz in {mrows.size} blocks, {kron_s * 1e3:.1f} ms, "
+        f"{kron_bytes / kron_s / 1e9:.0f} GB/s written")
+    # one closure step K * K with G at step_scale
+    mlen = np.zeros(64, np.int64)
+    mlen[mrows] = np.diff(mrpt.astype(np.int64))
+    two_step = np.add.reduceat(mlen[mcols], mrpt[:-1].astype(np.int64))  # sum over j in M_i of |M_j|
+    r, c = gen.rmat(step_scale, ef, seed)
+    hrpt, hrows, hcols = gen.to_dcsr(r, c)
+    m = 1 << step_scale
+    pg = row_products(hrpt, hrows, hcols, m, hrpt, hrows)
+    H = cb.matrix_dcsr_from_cpu(ctl, hrpt, hrows, hcols, m, m)
+    Mall = cb.matrix_dcsr_from_cpu(ctl, mrpt, mrows, mcols, 64, 64)
+    K = cb.dcsr.kronecker_product(ctl, Mall, H)
+    # K's stored rows are (i, g) in that order; row (i, g) has
+    # sum_{j in M_i} sum_{h in G_g} |M_j| |G_h| = two_step[i] * products(G_g) products
+    prod = np.concatenate([int(s) * pg for s in two_step])
+    blocks = row_blocks(prod, float(1 << 31))
+    step = {"graph_scale": step_scale, "k_nrows": 64 * m, "k_nnz": int(K.nnz), "products": int(prod.sum()),
+            "row_blocks": len(blocks), "step_products_at_full_scale": int(two_step.sum()) * g_products}
+    train = train_predictor(cb, gen, ctl, log)
+    prints = {}
+    for name, mode in (("plain", cb.SPGEMM_PLAIN), ("learned", cb.SPGEMM_LEARNED)):
+        best = None
+        for rep in range(3):
+            ctl.sync()
+            t = time.perf_counter()
+            nnz, fp = 0, []
+            for lo, hi in blocks:
+                Cb, st = cb.dcsr.spgemm(ctl, K, K, mode=mode, row_begin=lo, row_end=hi)
+                nnz += Cb.nnz
+                if rep == 2:
+                    fp.append(_fingerprint(ctl, Cb))
+                del Cb
+            ctl.sync()
+            dt = time.perf_counter() - t
+            if rep > 0:
+                best = dt if best is None else min(best, dt)
+        prints[name] = fp
+        step[name] = {"seconds": best, "products_per_s": float(prod.sum()) / best, "nnz": nnz}
+        log(f"[bench] C5 K*K step (G scale {step_scale}) {name}: {prod.sum():.3e} products in {best * 1e3:.1f} ms, "
+            f"{prod.sum() / best:.3e} products/s, nnz {nnz}")
+    step["identical_results"] = prints["plain"] == prints["learned"]
+    step["learned_vs_plain"] = step["plain"]["seconds"] / step["learned"]["seconds"]
+    step["predictor_training_ms"] = train["ms"]
+    out["kk_step"] = step
+    return out
+
+
 def bench_hypersparse(cb, gen, ctl, n_train, log, n_test=4, log_n=24):
     """The hypersparse learned-sizing workload: 2^24 x 2^24, 1% of rows
     nonempty, Zipf(1.5) row lengths capped at 4096, uniform columns; A * A.
@@ -312,6 +411,9 @@ def main():
                     help="also run the hypersparse learned-sizing workload, training on this many instances")
     ap.add_argument("--hypersparse-only", type=int, default=0,
                     help="run only the hypersparse workload, training on this many instances")
+    ap.add_argument("--c5-only", action="store_true",
+                    help="run only the automaton (x) R-MAT 22/32 workload: the Kronecker in automaton-row "
+                         "blocks and a bounded sample of one K * K closure step")
     ap.add_argument("--copy-probe", type=float, default=0,
                     help="only run a device-to-device copy of this many bytes (4 times): the reference "
                          "point that scales the FETCH_SIZE / WRITE_SIZE counters of a profiled run")
@@ -329,6 +431,12 @@ def main():
             dt = time.perf_counter() - t0
             log(f"[copy-probe] {nbytes} bytes in {dt * 1e3:.3f} ms ({2 * nbytes / dt / 1e9:.1f} GB/s read+write)")
         print(json.dumps({"copy_probe_bytes": nbytes}))
+        return
+    if args.c5_only:
+        import clbool_amd as cb
+        from clbool_amd import generators as gen
+        ctl = cb.create_controls()
+        print(json.dumps({"c5": bench_c5(cb, gen, ctl, log)}), flush=True)
         return
     if args.hypersparse_only:
         import clbool_amd as cb

@@ -168,6 +168,22 @@ def test_kronecker_automaton_times_graph(controls):
     assert_dcsr_equal(controls, out, oracle.pairs_to_dcsr(er, ec), "automaton kron")
 
 
+def test_kronecker_many_rows(controls):
+    """More output tiles than one launch takes (5M A rows times a B of one
+    nonzero and of three rows): the fill kernel runs in several launches of
+    whole A rows."""
+    n = 5 << 20
+    ar = np.arange(n, dtype=np.uint32)
+    ac = (ar * 7) % n
+    for br, bc in ((np.array([3], np.uint32), np.array([2], np.uint32)),
+                   (np.array([0, 0, 5], np.uint32), np.array([1, 4, 0], np.uint32))):
+        A, a = pairs_dcsr(controls, ar, ac, n, n)
+        B, b = pairs_dcsr(controls, br, bc, 8, 8)
+        out = cb.dcsr.kronecker_product(controls, a, b)
+        er, ec = oracle.kron(ar, ac, br, bc, 8, 8)
+        assert_dcsr_equal(controls, out, oracle.pairs_to_dcsr(er, ec), f"kron many rows, b nnz {br.size}")
+
+
 # ---- conversions (test_conversions.cpp, conversions.cpp:49-84)
 @pytest.mark.parametrize("size,k", [(200, 0), (200, 5), (4000, 30), (4700, 70), (29700, 40)])
 def test_dcsr_csr_round_trips(controls, size, k):
