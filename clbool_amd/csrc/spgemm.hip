@@ -589,7 +589,7 @@ void spgemm_plain(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &b, u32
         out.ncols = b.ncols;
         return;
     }
-    Out o1{nnz_row.ptr, nullptr, nullptr, nullptr, reinterpret_cast<u32 *>(s.scal.ptr + 3), Dump{}};
+    Out o1{nnz_row.ptr, nullptr, nullptr, nullptr, reinterpret_cast<u32 *>(s.scal.ptr + 3), Dump{}, 0};
     launch_bins(ctl, s, perm.ptr, h1.off, SYM, o1, &lt, -1);
     tm.mark();  // 2
     DevArray<u64> offs;
@@ -622,7 +622,7 @@ void spgemm_plain(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &b, u32
     if (nnz == 0) return;
     clb_buffer_s *cols = buffer_new(ctl, 4ull * nnz);
     try {
-        Out o2{nnz_row.ptr, offs.ptr, u32p(cols), nullptr, reinterpret_cast<u32 *>(s.scal.ptr + 3), Dump{}};
+        Out o2{nnz_row.ptr, offs.ptr, u32p(cols), nullptr, reinterpret_cast<u32 *>(s.scal.ptr + 3), Dump{}, 0};
         DumpBufs dump;
         setup_dump(ctl, s, h2, dump, o2);
         launch_bins(ctl, s, perm2.ptr, h2.off, NUM, o2, &lt, -1);

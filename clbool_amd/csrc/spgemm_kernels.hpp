@@ -372,12 +372,14 @@ __global__ __launch_bounds__(RS_WG) void k_row_stats(Mats g, u32 nrows, u32 *__r
 
 // Insert into an LDS open-addressing table of 2^BITS slots (linear probing,
 // EMPTY marks a free slot).  Returns 1 if col was new, 0 if present, and 2
-// if the table is full (only possible when a capacity guess was too small).
+// if the table is full (only possible when a capacity guess was too small)
+// or after `max_probe` probes.
 template <int BITS>
-__device__ __forceinline__ u32 hash_insert(u32 *table, u32 col) {
+__device__ __forceinline__ u32 hash_insert(u32 *table, u32 col, u32 max_probe) {
     constexpr u32 T = 1u << BITS;
     u32 h = dev::hash_slot(col, BITS);
-    for (u32 probe = 0; probe < T; ++probe) {
+    const u32 np = max_probe < T ? max_probe : T;
+    for (u32 probe = 0; probe < np; ++probe) {
         u32 cur = table[h];
         if (cur == EMPTY) {
             cur = atomicCAS(&table[h], EMPTY, col);
@@ -422,7 +424,14 @@ struct Out {
     const u32 *cap;
     u32 *overflow;  // SYM/NUM: bit flags of inconsistencies; ONE: count of failed rows
     Dump dump;
+    // ONE: a hash insertion that probes this many slots fails the row (0:
+    // no limit).  With a table of at least twice the capacity, a long probe
+    // sequence means the row holds more distinct columns than its capacity
+    // (the prediction was too small); stopping early keeps a badly sized row
+    // from probing a full table once per product.  The row is re-run exactly.
+    u32 probe_limit;
 };
+constexpr u32 ONE_PASS_PROBES = 64;
 
 template <int MODE>
 __device__ __forceinline__ u32 row_limit(const Out &o, u32 r) {
@@ -516,8 +525,9 @@ __global__ __launch_bounds__(WG) void k_hash(Mats g, const u32 *__restrict__ per
     for (u32 t = threadIdx.x; t < T; t += WG) table[t] = EMPTY;
     __syncthreads();
     u32 mine = 0, full = 0;
+    const u32 max_probe = MODE == ONE && o.probe_limit ? o.probe_limit : T;
     auto ins = [&](u32 col) {
-        const u32 rc = hash_insert<BITS>(table, col);
+        const u32 rc = hash_insert<BITS>(table, col, max_probe);
         mine += rc == 1;
         full |= rc == 2;
     };

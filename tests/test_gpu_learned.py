@@ -7,6 +7,8 @@
   over-predicting one.
 * What the trained network gains over sizing without it.
 """
+import time
+
 import numpy as np
 import pytest
 
@@ -141,6 +143,34 @@ def test_underpredicting_model_reruns_every_row_exactly(controls):
     assert st["fallback_rows"] <= int((lens > 8).sum())
 
 
+def test_midsize_underprediction_fails_fast_and_exact(controls):
+    """A model that predicts 2047 columns for every row puts rows with up to
+    a million distinct columns (over 2^22) into 2^13-slot LDS hash tables.
+    A full table must fail its row after a bounded probe sequence -- not probe
+    the whole table once per product -- and the failed rows are re-run
+    exactly."""
+    p = np.zeros(cb.MLP_PARAMS, np.float32)
+    p[-1] = 11.0 / 32.0  # log2(1 + N) / 32 = 11/32: N = 2047
+    cb.predictor_set_params(controls, p)
+    rng = np.random.default_rng(31)
+    hubs, ncols = 64, 1 << 22
+    lens = rng.choice([300, 3000, 12000, 40000], hubs)
+    rb = np.repeat(np.arange(hubs, dtype=np.uint32), lens)
+    cbv = rng.integers(0, ncols, rb.size).astype(np.uint32)
+    k = rng.integers(1, 31, 300)
+    ra = np.repeat(np.arange(300, dtype=np.uint32), k)
+    ca = rng.integers(0, hubs, ra.size).astype(np.uint32)
+    A, a = pairs_dcsr(controls, *np.unique(np.c_[ra, ca], axis=0).T, 300, hubs)
+    B, b = pairs_dcsr(controls, *np.unique(np.c_[rb, cbv], axis=0).T, hubs, ncols)
+    t0 = time.perf_counter()
+    got, st = cb.dcsr.spgemm(controls, a, b, mode=cb.SPGEMM_LEARNED)
+    controls.sync()
+    elapsed = time.perf_counter() - t0
+    assert_dcsr_equal(controls, got, oracle.spgemm(A, B), "mid-size under-prediction")
+    assert st["fallback_rows"] > 0
+    assert elapsed < 10.0, f"one pass with full hash tables took {elapsed:.1f} s"
+
+
 def test_overpredicting_model_is_exact(controls):
     p = np.zeros(cb.MLP_PARAMS, np.float32)
     p[-1] = 1.0  # predicts 2^32 - 1: every capacity clamps to min(P, ncols)
@@ -160,7 +190,6 @@ def test_trained_network_gains_over_sizing_without_it(controls, trained):
     columns span 2^22 (so LDS hash tables, not bitmaps, hold most rows) the
     trained model reserves far less memory than the bound, re-runs almost no
     rows, and is faster than re-running them."""
-    import time
     n, wide = 1 << 15, 1 << 22
     r, c = gen.rmat(15, 16, 31)
     A, a = pairs_dcsr(controls, r, c, n, n)
