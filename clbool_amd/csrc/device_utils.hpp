@@ -209,11 +209,30 @@ __device__ __forceinline__ uint32_t emit_step(uint32_t word, uint32_t cb, uint64
 // word's columns into the wave's staging area `sg` (STAGE_WORDS words), then
 // the wave copies them out in order, 64 consecutive columns per store
 // instruction.  Scattered per-lane stores would each touch their own cache
-// line; the staged copy writes whole lines.  Column t of the step sits at
-// sg[t + t / 32]: lanes write about 16 apart in a dense step, which without
-// the padding word per 32 would put all 64 lanes on two LDS banks.  Returns
-// the step's set bits.
+// line; the staged copy writes whole lines.  In a dense step (more than four
+// bits per word on average) lanes write about 16 apart, which would put all
+// 64 lanes on a few LDS banks, so column t then sits at sg[t + t / 32] (one
+// padding word per 32); sparser steps stage unpadded.  Returns the step's
+// set bits.
 constexpr uint32_t STAGE_WORDS = 2048 + 64;
+// Lane l copies columns l, l + 64, ... of n: column 64k + l sits at src[KS * k]
+// (src = the lane's first slot; KS = 66 padded, 64 unpadded) and goes to byte
+// 4 * (64k + l) of dst (a uniform pointer, so stores use a 32-bit offset).
+template <uint32_t KS>
+__device__ __forceinline__ void staged_copy_out(const uint32_t *src, char *dst, uint32_t l, uint32_t n) {
+    const uint32_t full = n / WAVE;
+    uint32_t k = 0, b = 4u * l;
+    for (; k + 4 <= full; k += 4, b += 4u * 4u * WAVE) {
+        const uint32_t v0 = src[KS * k], v1 = src[KS * (k + 1)], v2 = src[KS * (k + 2)], v3 = src[KS * (k + 3)];
+        *reinterpret_cast<uint32_t *>(dst + b) = v0;
+        *reinterpret_cast<uint32_t *>(dst + b + 4u * WAVE) = v1;
+        *reinterpret_cast<uint32_t *>(dst + b + 8u * WAVE) = v2;
+        *reinterpret_cast<uint32_t *>(dst + b + 12u * WAVE) = v3;
+    }
+    for (; k < full; ++k, b += 4u * WAVE) *reinterpret_cast<uint32_t *>(dst + b) = src[KS * k];
+    if (WAVE * k + l < n) *reinterpret_cast<uint32_t *>(dst + b) = src[KS * k];
+}
+
 __device__ __forceinline__ uint32_t emit_step_staged(uint32_t word, uint32_t cb, uint64_t pos, uint32_t *out,
                                                      uint64_t lim, uint32_t *sg) {
     const uint32_t l = (uint32_t)lane();
@@ -221,20 +240,34 @@ __device__ __forceinline__ uint32_t emit_step_staged(uint32_t word, uint32_t cb,
     const uint32_t inc = wave_incl_sum_dpp(pc);
     const uint32_t st = (uint32_t)__builtin_amdgcn_readlane((int)inc, 63);
     if (st == 0) return 0;
-    uint32_t w = word, o = inc - pc;
+    uint32_t w = word;
+    const uint32_t o = inc - pc;
     const uint32_t c0 = cb + 32u * l;
-    while (w) {
-        sg[o + (o >> 5)] = c0 + (uint32_t)__builtin_ctz(w);
-        ++o;
-        w &= w - 1u;
+    const bool padded = st > 4u * WAVE;  // uniform
+    if (padded) {
+        uint32_t q = o;
+        while (w) {
+            sg[q + (q >> 5)] = c0 + (uint32_t)__builtin_ctz(w);
+            ++q;
+            w &= w - 1u;
+        }
+    } else {
+        uint32_t *p = sg + o;
+        while (w) {
+            *p++ = c0 + (uint32_t)__builtin_ctz(w);
+            w &= w - 1u;
+        }
     }
     __builtin_amdgcn_wave_barrier();
-    uint32_t *dst = out + pos;
+    // the row's base as a uniform (scalar) pointer: stores take a 32-bit lane offset
+    const uint64_t up = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(pos >> 32)) << 32) |
+                        (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)pos);
+    char *dst = reinterpret_cast<char *>(out + up);
     // a row holds fewer than 2^32 columns
-    const uint32_t room = (uint32_t)(lim > pos ? lim - pos : 0);
+    const uint32_t room = (uint32_t)(lim > up ? lim - up : 0);
     const uint32_t n = st < room ? st : room;
-#pragma unroll 4
-    for (uint32_t t = l; t < n; t += (uint32_t)WAVE) dst[t] = sg[t + (t >> 5)];
+    if (padded) staged_copy_out<66>(sg + l + (l >> 5), dst, l, n);
+    else staged_copy_out<64>(sg + l, dst, l, n);
     __builtin_amdgcn_wave_barrier();
     return st;
 }
