@@ -177,8 +177,10 @@ __device__ __forceinline__ uint32_t emit_step(uint32_t word, uint32_t cb, uint64
         }
         return st;
     }
-    // the step fits: stores relative to one uniform base, no checks
-    uint32_t *o = out + pos;
+    // the step fits: stores relative to one uniform (scalar) base, no checks
+    const uint64_t up = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(pos >> 32)) << 32) |
+                        (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)pos);
+    uint32_t *o = out + up;
     if (st >= dense) {
         uint32_t p = 0;
         const uint32_t sh = l & 31u;
@@ -189,7 +191,8 @@ __device__ __forceinline__ uint32_t emit_step(uint32_t word, uint32_t cb, uint64
             const uint32_t v = l < 32 ? w0 : w1;
             const bool set = (v >> sh) & 1u;
             const uint64_t m = __ballot(set);
-            if (set) o[p + lanes_below(m)] = cb + 64u * k + l;
+            if (set) *reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(o) + 4u * (p + lanes_below(m))) =
+                         cb + 64u * k + l;
             p += (uint32_t)__popcll(m);
         }
     } else {
