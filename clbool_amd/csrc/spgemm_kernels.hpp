@@ -431,7 +431,7 @@ struct Out {
     // from probing a full table once per product.  The row is re-run exactly.
     u32 probe_limit;
 };
-constexpr u32 ONE_PASS_PROBES = 64;
+constexpr u32 ONE_PASS_PROBES = 256;
 
 template <int MODE>
 __device__ __forceinline__ u32 row_limit(const Out &o, u32 r) {
@@ -516,13 +516,18 @@ __global__ __launch_bounds__(WAVES * 64) void k_tiny(Mats g, const u32 *__restri
 template <int BITS, int WG, int MODE>
 __global__ __launch_bounds__(WG) void k_hash(Mats g, const u32 *__restrict__ perm, Out o) {
     constexpr u32 T = 1u << BITS;
-    __shared__ u32 table[T];
+    __shared__ __align__(16) u32 table[T];
     __shared__ __align__(8) u32 big[BIG_WORDS(WG)];
     __shared__ u32 red[WG / 64 + 1];
     const u32 r = perm[blockIdx.x];
     const u32 i = g.row0 + r;
     const u32 js = g.a_rpt[i], je = g.a_rpt[i + 1];
-    for (u32 t = threadIdx.x; t < T; t += WG) table[t] = EMPTY;
+    if (T >= 4 * WG) {
+        uint4 *t4 = reinterpret_cast<uint4 *>(table);
+        for (u32 t = threadIdx.x; t < T / 4; t += WG) t4[t] = make_uint4(EMPTY, EMPTY, EMPTY, EMPTY);
+    } else {
+        for (u32 t = threadIdx.x; t < T; t += WG) table[t] = EMPTY;
+    }
     __syncthreads();
     u32 mine = 0, full = 0;
     const u32 max_probe = MODE == ONE && o.probe_limit ? o.probe_limit : T;
@@ -555,12 +560,14 @@ __global__ __launch_bounds__(WG) void k_hash(Mats g, const u32 *__restrict__ per
         atomicOr(o.overflow, 1u);
     }
     // compact the live slots to the front of the table
+    // (strided slots: consecutive lanes read consecutive words, no bank
+    // conflicts; the packed order does not matter, the sort follows)
     constexpr u32 PER = T / WG > 0 ? T / WG : 1;
     u32 v[PER];
     u32 c = 0;
 #pragma unroll
     for (u32 q = 0; q < PER; ++q) {
-        const u32 t = threadIdx.x * PER + q;
+        const u32 t = q * WG + threadIdx.x;
         v[q] = t < T ? table[t] : EMPTY;
         c += v[q] != EMPTY;
     }
