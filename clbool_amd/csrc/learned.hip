@@ -6,8 +6,8 @@
 // sizes the row's LDS table and picks its bin, and a single numeric pass
 // writes every row into a temporary buffer at predicted offsets.  A row
 // with more distinct columns than its predicted capacity is re-run exactly
-// (capacity min(P, ncols), which cannot overflow), so the exact path stays
-// both the fallback and the judge.  The final DCSR is compacted from the
+// (capacity min(P, ncols), which cannot overflow), so every row's result is
+// exact whatever the prediction.  The final DCSR is compacted from the
 // temporary buffers with exact offsets.  The only precedent in the reference
 // is the unused overflow-and-record-failed-rows scheme of hash_large.cl:103-131.
 //
