@@ -364,7 +364,7 @@ void spgemm_learned(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &b, u
     u32 *fails = reinterpret_cast<u32 *>(s.scal.ptr + 5);
     Out o{nnz_row.ptr, toff.ptr, temp.ptr, cap.ptr, fails, Dump{}, ONE_PASS_PROBES};
     DumpBufs dump;
-    setup_dump(ctl, s, h, dump, o);
+    setup_dump(ctl, s, h, dump, o, true);
     launch_bins(ctl, s, perm.ptr, h.off, ONE, o, &lt, CLB_KP_ONEPASS);
     u32 nfail = 0;
     readback(ctl, &nfail, fails, 4);

@@ -178,10 +178,13 @@ __global__ __launch_bounds__(EMIT_WG) void k_bitmap_emit(const uint4 *__restrict
 }
 
 // ---------------------------------------------------------------------------
-void setup_dump(clb_controls_s *ctl, const Setup &s, const BinHost &h, DumpBufs &d, Out &o) {
+void setup_dump(clb_controls_s *ctl, const Setup &s, const BinHost &h, DumpBufs &d, Out &o, bool one_pass) {
     o.dump = Dump{};
     d.on = false;
-    const u32 knob = ctl->tuning.dump_bits_per_chunk ? ctl->tuning.dump_bits_per_chunk : 256u;
+    // default: on in the one pass (its in-kernel emission writes a temporary
+    // slot that the compaction copies again), off in the numeric pass (which
+    // writes the final positions directly)
+    const u32 knob = ctl->tuning.dump_bits_per_chunk ? ctl->tuning.dump_bits_per_chunk : (one_pass ? 1024u : 65536u);
     if (s.cfg.bm_bits < 14 || knob >= 65536u) return;
     const u64 nslices = ((u64)s.g.ncols + (32ull << s.cfg.bm_bits) - 1) / (32ull << s.cfg.bm_bits);
     d.nbm = h.off[NBINS] - h.off[BIN_BITMAP0];
@@ -624,7 +627,7 @@ void spgemm_plain(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &b, u32
     try {
         Out o2{nnz_row.ptr, offs.ptr, u32p(cols), nullptr, reinterpret_cast<u32 *>(s.scal.ptr + 3), Dump{}, 0};
         DumpBufs dump;
-        setup_dump(ctl, s, h2, dump, o2);
+        setup_dump(ctl, s, h2, dump, o2, false);
         launch_bins(ctl, s, perm2.ptr, h2.off, NUM, o2, &lt, -1);
         emit_dense(ctl, perm2.ptr, h2, dump, offs.ptr, u32p(cols), &lt, CLB_KP_NUMERIC, timing ? &st : nullptr);
         tm.mark();  // 4

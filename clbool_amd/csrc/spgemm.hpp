@@ -57,7 +57,7 @@ struct DumpBufs {
 // Allocates the dense-bitmap scratch for the bitmap bin of `h` (nothing when
 // the window is below 2^14 words, the knob disables it or it would not fit)
 // and fills o.dump.
-void setup_dump(clb_controls_s *ctl, const Setup &s, const BinHost &h, DumpBufs &d, Out &o);
+void setup_dump(clb_controls_s *ctl, const Setup &s, const BinHost &h, DumpBufs &d, Out &o, bool one_pass);
 // Expands the dumped rows into cols at the exact row offsets `off`.
 void emit_dense(clb_controls_s *ctl, const u32 *perm, const BinHost &h, DumpBufs &d, const u64 *off, u32 *cols,
                 LaunchTimer *lt, int pass, clb_spgemm_stats *st);

@@ -72,10 +72,12 @@ typedef struct clb_tuning {
                                       this many set bits are written one lane per
                                       bit, sparser ones one lane per word
                                       (0: 1024)                                   */
-    uint32_t dump_bits_per_chunk;  /* rows over one full 2^20-column window with
-                                      at least this many set bits per 1024 words
-                                      are dumped and expanded by a separate
-                                      emission kernel (0: 256; >= 65536: off)     */
+    uint32_t dump_bits_per_chunk;  /* bitmap rows (windows of at least 2^14
+                                      words) whose first window holds at least
+                                      this many set bits per 1024 words are
+                                      dumped and expanded by a separate emission
+                                      kernel (>= 65536: off; 0: 1024 in the
+                                      learned one pass, off in the numeric pass) */
     uint32_t bitmap_persistent;    /* nonzero: the bitmap rows run on a persistent
                                       grid that takes rows from a counter
                                       instead of one workgroup per row          */
