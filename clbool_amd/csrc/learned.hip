@@ -365,6 +365,7 @@ void spgemm_learned(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &b, u
     Out o{nnz_row.ptr, toff.ptr, temp.ptr, cap.ptr, fails, Dump{}, ONE_PASS_PROBES};
     DumpBufs dump;
     setup_dump(ctl, s, h, dump, o, true);
+    ensure_splits(ctl, s, h);
     launch_bins(ctl, s, perm.ptr, h.off, ONE, o, &lt, CLB_KP_ONEPASS);
     u32 nfail = 0;
     readback(ctl, &nfail, fails, 4);
@@ -401,6 +402,7 @@ void spgemm_learned(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &b, u
         side.reset(ctl, side_total ? side_total : 1);
         u32 *fails2 = reinterpret_cast<u32 *>(s.scal.ptr + 7);
         Out o2{nnz_row.ptr, soff.ptr, side.ptr, key2.ptr, fails2, Dump{}, 0};  // exact: full probing
+        ensure_splits(ctl, sx, h2);
         launch_bins(ctl, sx, perm2.ptr, h2.off, ONE, o2, &lt, CLB_KP_FALLBACK);
         sx.cfg = keep;
         u32 nfail2 = 0;
@@ -536,6 +538,7 @@ clb_status clb_predictor_harvest(clb_controls ctl, const clb_dcsr *a, const clb_
             bin_rows(ctl, s, nullptr, nnz_row.ptr, perm, bb);
             read_bins(ctl, bb, h);
             Out o{nnz_row.ptr, nullptr, nullptr, nullptr, reinterpret_cast<u32 *>(s.scal.ptr + 3), Dump{}, 0};
+            ensure_splits(ctl, s, h);
             launch_bins(ctl, s, perm.ptr, h.off, SYM, o, nullptr, -1);
             const u32 stride = nrows > max_rows ? div_up(nrows, max_rows) : 1;
             const u32 take = div_up(nrows, stride);

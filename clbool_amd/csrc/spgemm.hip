@@ -260,6 +260,24 @@ BinCfg make_bincfg(const clb_controls_s *ctl, u32 ncols) {
     return c;
 }
 
+// The per-slice split table of B (Mats::splits) for bitmap rows over
+// several column windows: built on the first pass that has bitmap rows.
+void ensure_splits(clb_controls_s *ctl, Setup &s, const BinHost &h) {
+    if (s.g.splits || h.off[NBINS] == h.off[BIN_BITMAP0]) return;
+    const clb_dcsr &b = s.b;
+    const u64 span = 32ull << s.cfg.bm_bits;
+    const u64 nsl = ((u64)b.ncols + span - 1) / span;
+    if (nsl <= 1 || !b.nzr || (u64)s.mapn * (nsl + 1) >= (1ull << 32)) return;
+    // written for stored B rows only: the gather reads a row's entries only
+    // when its range in brange is nonempty
+    s.splits.reset(ctl, (u64)s.mapn * (nsl + 1));
+    k_slice_splits<<<div_up(b.nzr, 256), 256, 0, ctl->stream>>>(u32p(b.rpt), u32p(b.rows), u32p(b.cols), b.nzr,
+                                                                 s.mapn, (u32)nsl, span, s.splits.ptr);
+    CLB_LAUNCHED("slice_splits");
+    s.g.splits = s.splits.ptr;
+    s.g.nsl = (u32)nsl;
+}
+
 void prepare(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &b, u32 row_begin, u32 nrows, Setup &s,
              bool count) {
     s.nrows = nrows;
@@ -287,15 +305,7 @@ void prepare(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &b, u32 row_
     s.g.splits = nullptr;
     s.g.nsl = 0;
     s.g.split_bits = s.cfg.bm_bits;
-    if (nsl > 1 && b.nzr && (u64)s.mapn * (nsl + 1) < (1ull << 32)) {
-        s.splits.reset(ctl, (u64)s.mapn * (nsl + 1));
-        fill_u32(ctl, s.splits.ptr, s.splits.n, 0u);
-        k_slice_splits<<<div_up(b.nzr, 256), 256, 0, ctl->stream>>>(u32p(b.rpt), u32p(b.rows), u32p(b.cols), b.nzr,
-                                                                     s.mapn, (u32)nsl, span, s.splits.ptr);
-        CLB_LAUNCHED("slice_splits");
-        s.g.splits = s.splits.ptr;
-        s.g.nsl = (u32)nsl;
-    }
+    s.b = b;  // the split table is built by ensure_splits, once bitmap rows exist
     s.prod.reset(ctl, nrows);
     s.ctr.reset(ctl, 4);
     s.scal.reset(ctl, 8);  // [0] total products, [1] nnz total, [2] nzr total, [3] overflow flag
@@ -593,6 +603,7 @@ void spgemm_plain(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &b, u32
         return;
     }
     Out o1{nnz_row.ptr, nullptr, nullptr, nullptr, reinterpret_cast<u32 *>(s.scal.ptr + 3), Dump{}, 0};
+    ensure_splits(ctl, s, h1);
     launch_bins(ctl, s, perm.ptr, h1.off, SYM, o1, &lt, -1);
     tm.mark();  // 2
     DevArray<u64> offs;
@@ -628,6 +639,7 @@ void spgemm_plain(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &b, u32
         Out o2{nnz_row.ptr, offs.ptr, u32p(cols), nullptr, reinterpret_cast<u32 *>(s.scal.ptr + 3), Dump{}, 0};
         DumpBufs dump;
         setup_dump(ctl, s, h2, dump, o2, false);
+        ensure_splits(ctl, s, h2);
         launch_bins(ctl, s, perm2.ptr, h2.off, NUM, o2, &lt, -1);
         emit_dense(ctl, perm2.ptr, h2, dump, offs.ptr, u32p(cols), &lt, CLB_KP_NUMERIC, timing ? &st : nullptr);
         tm.mark();  // 4

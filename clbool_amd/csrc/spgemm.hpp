@@ -18,6 +18,7 @@ struct Setup {
     DevArray<u32> splits;  // per-slice B row ranges (Mats::splits) when ncols spans several windows
     Mats g{};
     BinCfg cfg{};
+    clb_dcsr b{};  // the B operand (its buffers stay owned by the caller)
 };
 
 // HIP-event timing of individual kernel launches, grouped by family
@@ -63,6 +64,7 @@ void emit_dense(clb_controls_s *ctl, const u32 *perm, const BinHost &h, DumpBufs
                 LaunchTimer *lt, int pass, clb_spgemm_stats *st);
 
 BinCfg make_bincfg(const clb_controls_s *ctl, u32 ncols);
+void ensure_splits(clb_controls_s *ctl, Setup &s, const BinHost &h);
 // B row map, kernel arguments and (count == true) per-row product counts.
 void prepare(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &b, u32 row_begin, u32 nrows, Setup &s,
              bool count = true);

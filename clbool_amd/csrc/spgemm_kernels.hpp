@@ -126,16 +126,16 @@ __device__ __forceinline__ void for_each_product(const Mats &g, u32 js, u32 je, 
         if (j < je) {
             const u32 k = g.a_cols[j];
             if (k < g.mapn) {
-                if (slice != NOSLICE) {
+                const uint2 rr = g.brange[k];
+                if (slice == NOSLICE) {
+                    bx = rr.x;
+                    len = rr.y - rr.x;
+                } else if (rr.y > rr.x) {  // split entries exist for stored B rows only
                     // fine slices [slice * sstep, (slice + 1) * sstep) of the row
                     const u32 *sp = g.splits + (u64)k * (g.nsl + 1);
                     const u32 f0 = slice * sstep, f1 = f0 + sstep < g.nsl ? f0 + sstep : g.nsl;
                     bx = sp[f0];
                     len = sp[f1] - bx;
-                } else {
-                    const uint2 r = g.brange[k];
-                    bx = r.x;
-                    len = r.y - r.x;
                 }
             }
         }
