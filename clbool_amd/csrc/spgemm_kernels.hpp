@@ -296,24 +296,25 @@ __global__ __launch_bounds__(RS_WG) void k_row_stats(Mats g, u32 nrows, u32 *__r
                                                      u32 *__restrict__ lc) {
     __shared__ u32 rs[RS_WG + 2];
     __shared__ u32 sp[RS_WG + 1], smn[RS_WG + 1], smx[RS_WG + 1], sfc[RS_WG + 1], slc[RS_WG + 1];
-    __shared__ u32 r0s;
     __shared__ u64 red[RS_WG / 64 + 1];
     const u32 rbeg = g.row0, rend = g.row0 + nrows;
     const u32 jbase = g.a_rpt[rbeg], jend = g.a_rpt[rend];
     const u64 j0 = (u64)jbase + (u64)blockIdx.x * RS_WG;
     if (j0 >= jend) return;  // uniform: the grid covers all of A's nonzeros
-    if (threadIdx.x == 0) {
-        // last row r in [rbeg, rend) with a_rpt[r] <= j0
-        u32 lo = rbeg, hi = rend;
-        while (hi - lo > 1) {
-            const u32 mid = lo + (hi - lo) / 2;
-            if (g.a_rpt[mid] <= j0) lo = mid;
-            else hi = mid;
-        }
-        r0s = lo;
+    // last row r in [rbeg, rend) with a_rpt[r] <= j0: rounds of a 256-way
+    // split of [lo, hi), one probe per thread (a_rpt[lo] <= j0 < a_rpt[hi]
+    // throughout); three rounds cover 2^24 rows, where one thread's binary
+    // search would be a chain of 24 dependent loads
+    u32 lo = rbeg, hi = rend;
+    while (hi - lo > 1) {
+        const u32 step = (hi - lo + RS_WG - 1) / RS_WG;
+        const u64 c = (u64)lo + (u64)threadIdx.x * step;
+        const bool ok = c < hi && g.a_rpt[c] <= j0;  // true for a prefix of the threads
+        const u32 cnt = (u32)__syncthreads_count(ok);  // >= 1: the probe at lo holds
+        lo += (cnt - 1) * step;
+        hi = hi - lo < step ? hi : lo + step;
     }
-    __syncthreads();
-    const u32 r0 = r0s;
+    const u32 r0 = lo;
     // rows r0 .. r0 + nr - 1 can hold this workgroup's nonzeros (every stored
     // row has one, so at most RS_WG + 1 of them)
     const u32 nr = rend - r0 < RS_WG + 1 ? rend - r0 : RS_WG + 1;
