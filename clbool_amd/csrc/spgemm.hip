@@ -43,11 +43,13 @@ __global__ void k_slice_splits(const u32 *__restrict__ b_rpt, const u32 *__restr
 }
 
 __global__ void k_brange_set(const u32 *__restrict__ b_rpt, const u32 *__restrict__ b_rows, u32 b_nzr, u32 mapn,
-                             uint2 *__restrict__ brange) {
+                             uint2 *__restrict__ brange, u32 *__restrict__ bpres) {
     const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= b_nzr) return;
     const u32 r = b_rows[i];
-    if (r < mapn) brange[r] = make_uint2(b_rpt[i], b_rpt[i + 1]);
+    if (r >= mapn) return;
+    brange[r] = make_uint2(b_rpt[i], b_rpt[i + 1]);
+    if (bpres) atomicOr(&bpres[r >> 5], 1u << (r & 31));
 }
 
 // Histogram of bins.  key == nullptr: key = min(P, ncols) (symbolic sizing).
@@ -284,12 +286,21 @@ void prepare(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &b, u32 row_
     s.row0 = row_begin;
     s.mapn = b.nrows;
     s.brange.reset(ctl, s.mapn ? s.mapn : 1);
-    fill_u64(ctl, reinterpret_cast<u64 *>(s.brange.ptr), s.mapn, 0ull);
+    // hypersparse B (under one stored row in 16): a presence bit per row id
+    // instead of clearing the whole row map
+    const bool sparse_map = (u64)b.nzr * 16 < s.mapn;
+    if (sparse_map) {
+        s.bpres.reset(ctl, ((u64)s.mapn + 31) / 32);
+        fill_u32(ctl, s.bpres.ptr, s.bpres.n, 0u);
+    } else {
+        fill_u64(ctl, reinterpret_cast<u64 *>(s.brange.ptr), s.mapn, 0ull);
+    }
     if (b.nzr) {
         k_brange_set<<<div_up(b.nzr, 256), 256, 0, ctl->stream>>>(u32p(b.rpt), u32p(b.rows), b.nzr, s.mapn,
-                                                                   s.brange.ptr);
+                                                                   s.brange.ptr, sparse_map ? s.bpres.ptr : nullptr);
         CLB_LAUNCHED("brange_set");
     }
+    s.g.bpres = sparse_map ? s.bpres.ptr : nullptr;
     s.g.a_rpt = u32p(a.rpt);
     s.g.a_rows = u32p(a.rows);
     s.g.a_cols = u32p(a.cols);

@@ -12,6 +12,7 @@ namespace spg {
 struct Setup {
     u32 nrows = 0, row0 = 0, mapn = 0;
     DevArray<uint2> brange;
+    DevArray<u32> bpres;  // presence bits of B rows (hypersparse B only)
     DevArray<u32> prod;
     DevArray<u64> scal;
     DevArray<u32> ctr;  // row counters of persistent launches (zeroed per launch)

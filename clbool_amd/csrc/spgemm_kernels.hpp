@@ -54,7 +54,19 @@ struct Mats {
     const u32 *splits;
     u32 nsl;
     u32 split_bits;
+    // Hypersparse B (few stored rows): one bit per B row id, set for stored
+    // rows.  A lookup tests the bit (a 2 MB table for 2^24 rows stays in L2)
+    // and reads brange only for stored rows; brange is written only there.
+    // nullptr: brange is complete.
+    const u32 *bpres;
 };
+
+// [begin, end) of B row k in b_cols ({0, 0} for absent rows).
+__device__ __forceinline__ uint2 b_range(const Mats &g, u32 k) {
+    if (k >= g.mapn) return make_uint2(0, 0);
+    if (g.bpres && !((g.bpres[k >> 5] >> (k & 31)) & 1u)) return make_uint2(0, 0);
+    return g.brange[k];
+}
 constexpr u32 NOSLICE = 0xFFFFFFFFu;
 
 // Bitmap emission.  Each wave owns a contiguous range of the bitmap's words
@@ -125,18 +137,16 @@ __device__ __forceinline__ void for_each_product(const Mats &g, u32 js, u32 je, 
         u32 bx = 0, len = 0;
         if (j < je) {
             const u32 k = g.a_cols[j];
-            if (k < g.mapn) {
-                const uint2 rr = g.brange[k];
-                if (slice == NOSLICE) {
-                    bx = rr.x;
-                    len = rr.y - rr.x;
-                } else if (rr.y > rr.x) {  // split entries exist for stored B rows only
-                    // fine slices [slice * sstep, (slice + 1) * sstep) of the row
-                    const u32 *sp = g.splits + (u64)k * (g.nsl + 1);
-                    const u32 f0 = slice * sstep, f1 = f0 + sstep < g.nsl ? f0 + sstep : g.nsl;
-                    bx = sp[f0];
-                    len = sp[f1] - bx;
-                }
+            const uint2 rr = b_range(g, k);
+            if (slice == NOSLICE) {
+                bx = rr.x;
+                len = rr.y - rr.x;
+            } else if (rr.y > rr.x) {  // split entries exist for stored B rows only
+                // fine slices [slice * sstep, (slice + 1) * sstep) of the row
+                const u32 *sp = g.splits + (u64)k * (g.nsl + 1);
+                const u32 f0 = slice * sstep, f1 = f0 + sstep < g.nsl ? f0 + sstep : g.nsl;
+                bx = sp[f0];
+                len = sp[f1] - bx;
             }
         }
         // list entry e: big[3e] = first B position, big[3e+1] = end,
@@ -340,8 +350,8 @@ __global__ __launch_bounds__(RS_WG) void k_row_stats(Mats g, u32 nrows, u32 *__r
             else hi = mid;
         }
         const u32 k = g.a_cols[j];
-        if (k < g.mapn) {
-            const uint2 b = g.brange[k];
+        {
+            const uint2 b = b_range(g, k);
             const u32 len = b.y - b.x;
             if (len) {
                 mine = len;
@@ -455,12 +465,9 @@ __global__ __launch_bounds__(WAVES * 64) void k_tiny(Mats g, const u32 *__restri
         const u32 j = base + l;
         u32 bx = 0, len = 0;
         if (j < je) {
-            const u32 k = g.a_cols[j];
-            if (k < g.mapn) {
-                const uint2 rr = g.brange[k];
-                bx = rr.x;
-                len = rr.y - rr.x;
-            }
+            const uint2 rr = b_range(g, g.a_cols[j]);
+            bx = rr.x;
+            len = rr.y - rr.x;
         }
         const u32 inc = dev::wave_incl_scan(len);
         const u32 tot = __shfl(inc, 63, 64);
