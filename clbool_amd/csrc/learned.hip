@@ -425,19 +425,25 @@ void spgemm_learned(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &b, u
     if (nnz == 0) return;
     clb_buffer_s *cols = buffer_new(ctl, 4ull * nnz);
     try {
-        lt.begin(ctl->stream);
-        k_compact_rows<<<nrows, 256, 0, ctl->stream>>>(nnz_row.ptr, offs.ptr, temp.ptr, toff.ptr, side.ptr, soff.ptr,
-                                                       nfail ? key2.ptr : nullptr, dump.on ? dump.isdense.ptr : nullptr,
-                                                       pred.ptr, u32p(cols), err.ptr);
+        // the compaction (rows from the temporary slots) and the dense-row
+        // emission write disjoint rows: they run side by side
+        fork_side(ctl);
+        hipStream_t cs = side_stream(ctl, 0);
+        lt.begin(cs);
+        k_compact_rows<<<nrows, 256, 0, cs>>>(nnz_row.ptr, offs.ptr, temp.ptr, toff.ptr, side.ptr, soff.ptr,
+                                              nfail ? key2.ptr : nullptr, dump.on ? dump.isdense.ptr : nullptr,
+                                              pred.ptr, u32p(cols), err.ptr);
         CLB_LAUNCHED("compact_rows");
-        lt.end((int)CLB_KF(CLB_KP_FALLBACK, CLB_KK_EMIT), ctl->stream);
+        lt.end((int)CLB_KF(CLB_KP_FALLBACK, CLB_KK_EMIT), cs);
         emit_dense(ctl, perm.ptr, h, dump, offs.ptr, u32p(cols), &lt, CLB_KP_ONEPASS, timing ? &st : nullptr);
+        join_side(ctl);
         assemble(ctl, s, a, nnz_row.ptr, offs.ptr, pos.ptr, nnz, nzr, cols, out);
         double e = 0;
         readback(ctl, &e, err.ptr, 8);
         st.pred_abs_log2_err = nzr ? e / nzr : 0.0;
     } catch (...) {
         (void)hipStreamSynchronize(ctl->stream);
+        for (int k = 0; k < clb_controls_s::NSIDE; ++k) (void)hipStreamSynchronize(ctl->side[k]);
         clb_dcsr_release(&out);
         if (!out.cols) buffer_release(cols);
         throw;
