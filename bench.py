@@ -52,11 +52,15 @@ def row_products(rpt, rows, cols, nrows_b, b_rpt, b_rows):
     return np.add.reduceat(pe, rpt[:-1].astype(np.int64)) if rows.size else np.zeros(0, np.int64)
 
 
-def row_blocks(prod, budget):
-    """Split stored rows into consecutive blocks of at most `budget` products."""
+def row_blocks(prod, budget, ncols=None):
+    """Split stored rows into consecutive blocks whose per-row products add up
+    to at most `budget`; with `ncols`, each row counts min(products, ncols),
+    an upper bound of its output size (a budget below 2^32 then guarantees
+    that every block's result fits uint32 indices)."""
+    w = np.minimum(prod, ncols) if ncols else prod
     bounds = [0]
     acc = 0
-    for i, p in enumerate(prod.tolist()):
+    for i, p in enumerate(w.tolist()):
         if acc + p > budget and i > bounds[-1]:
             bounds.append(i)
             acc = 0
@@ -65,8 +69,10 @@ def row_blocks(prod, budget):
     return list(zip(bounds[:-1], bounds[1:]))
 
 
-def validate_sample(cb, oracle, ctl, a_dev, A, blocks, mode, rng, rows_per_block=48):
-    """Bit-exact check of sampled rows of every block against the oracle."""
+def validate_sample(cb, oracle, ctl, a_dev, A, blocks, mode, rng, rows=480):
+    """Bit-exact check of sampled rows of every block against the oracle
+    (about `rows` rows in all, at least 48 per block)."""
+    rows_per_block = max(48, -(-rows // max(1, len(blocks))))
     checked = 0
     for lo, hi in blocks:
         C, st = cb.dcsr.spgemm(ctl, a_dev, a_dev, mode=mode, row_begin=lo, row_end=hi)
@@ -211,7 +217,7 @@ def bench_c5(cb, gen, ctl, log, scale=22, ef=32, step_scale=16, seed=1):
     One closure step K * K at this scale is products(M*M) * products(G*G),
     about 3e14 products, beyond one GPU; the step itself is measured on the
     same automaton with G at scale `step_scale` (K of width 64 * 2^16 = 2^22),
-    whole, in row blocks of at most 2^31 products, plain and learned; both
+    whole, in row blocks of output bound below 2^32, plain and learned; both
     results must agree."""
     mr, mc = gen.uniform_pairs(200, 64, 64, seed)
     mrpt, mrows, mcols = gen.to_dcsr(mr, mc)
@@ -260,7 +266,7 @@ def bench_c5(cb, gen, ctl, log, scale=22, ef=32, step_scale=16, seed=1):
     # K's stored rows are (i, g) in that order; row (i, g) has
     # sum_{j in M_i} sum_{h in G_g} |M_j| |G_h| = two_step[i] * products(G_g) products
     prod = np.concatenate([int(s) * pg for s in two_step])
-    blocks = row_blocks(prod, float(1 << 31))
+    blocks = row_blocks(prod, float((1 << 32) - 1), ncols=64 * m)
     step = {"graph_scale": step_scale, "k_nrows": 64 * m, "k_nnz": int(K.nnz), "products": int(prod.sum()),
             "row_blocks": len(blocks), "step_products_at_full_scale": int(two_step.sum()) * g_products}
     train = train_predictor(cb, gen, ctl, log)
@@ -398,7 +404,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="rmat20", choices=sorted(WORKLOADS))
     ap.add_argument("--mode", default="auto", choices=["auto", "plain", "learned"])
-    ap.add_argument("--block-products", type=float, default=float(1 << 31))
+    ap.add_argument("--block-bound", type=float, default=float((1 << 32) - 1),
+                    help="largest output bound (sum of min(products, ncols) over its rows) of one row block")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-validate", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
@@ -462,7 +469,7 @@ def main():
     dev = cb.show_devices()[0]
     a_dev = cb.matrix_dcsr_from_cpu(ctl, rpt, rows, cols, n, n)
     prod = row_products(rpt, rows, cols, n, rpt, rows)
-    blocks = row_blocks(prod, int(args.block_products))
+    blocks = row_blocks(prod, int(args.block_bound), ncols=n)
     total_products = int(prod.sum())
     log(f"[bench] A nnz={cols.size} nzr={rows.size} products={total_products} blocks={len(blocks)}")
 

@@ -9,7 +9,7 @@
 // by t * n) times Z' = [Z_0; Z_1; ...] (row block t holds Z_t) gives row block
 // t of the result = Y_t * Z_t, so every kernel launch of the SpGEMM covers the
 // rows of all products.  The product is taken in row blocks of Y' whose
-// output bound (sum over rows of min(products, n)) stays below 2^31, so each
+// output bound (sum over rows of min(products, n)) stays below 2^32, so each
 // block's result fits uint32 indices however large the step grows.  Each X's
 // union merges sorted key segments -- its own keys and, per triple that
 // targets it, that triple's product rows (already sorted: a product block is
@@ -256,7 +256,7 @@ clb_status clb_closure_step(clb_controls ctl, clb_dcsr *mats, uint32_t nmats, co
             try {
                 Z = stack(ctl, zs, ntriples * n, n);
                 if (Y.nnz && Z.nnz) {
-                    // row blocks of Y' with output bound below 2^31 each
+                    // row blocks of Y' with output bound below 2^32 each
                     DevArray<u32> zlen(ctl, (u64)ntriples * n);
                     fill_u32(ctl, zlen.ptr, (u64)ntriples * n, 0u);
                     k_row_lengths<<<div_up(Z.nzr, 256), 256, 0, ctl->stream>>>(u32p(Z.rpt), u32p(Z.rows), Z.nzr,
@@ -269,7 +269,7 @@ clb_status clb_closure_step(clb_controls ctl, clb_dcsr *mats, uint32_t nmats, co
                     exclusive_scan_u64(ctl, pre.ptr, pre.ptr, Y.nzr, tot.ptr);
                     u64 total = 0;
                     readback(ctl, &total, tot.ptr, 8);
-                    const u64 lim = o.block_bound ? o.block_bound : 1ull << 31;
+                    const u64 lim = o.block_bound && o.block_bound < 0xFFFFFFFFull ? o.block_bound : 0xFFFFFFFFull;
                     const u32 nb = (u32)((total + lim - 1) / lim);
                     DevArray<u32> cuts(ctl, (u64)nb + 1);
                     k_cuts<<<div_up(nb + 1, 256), 256, 0, ctl->stream>>>(pre.ptr, Y.nzr, lim, nb, cuts.ptr);

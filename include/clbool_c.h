@@ -249,7 +249,7 @@ typedef struct clb_closure_opts {
     uint32_t mode;         /* CLB_CLOSURE_*                                   */
     uint32_t spgemm_mode;  /* CLB_SPGEMM_PLAIN or CLB_SPGEMM_LEARNED           */
     uint32_t block_bound;  /* batched: cut the stacked product into row blocks
-                              whose output bound stays below this (0: 2^31)   */
+                              whose output bound stays below this (0: 2^32 - 1) */
     uint32_t reserved;
 } clb_closure_opts;
 typedef struct clb_closure_stats {
