@@ -200,7 +200,9 @@ void setup_dump(clb_controls_s *ctl, const Setup &s, const BinHost &h, DumpBufs 
         (void)hipGetLastError();
         return;
     }
-    if (bytes > free_b / 3) return;  // keep most of HBM for the operands and the output
+    // keep most of HBM for the operands and the output; blocks cached by the
+    // pool count as free (the pool hands them back when an allocation fails)
+    if (bytes > (free_b + pool_for(ctl->device).cached_bytes()) / 3) return;
     d.store.reset(ctl, (u64)d.nbm * d.words / 4);
     d.cnt.reset(ctl, (u64)d.nbm * d.nch);
     d.list.reset(ctl, d.nbm);
