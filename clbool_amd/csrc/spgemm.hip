@@ -136,7 +136,7 @@ __global__ void k_set_positions(const u32 *__restrict__ nnz_row, const u64 *__re
 // their final positions.  Each of the 4 waves owns a contiguous quarter of
 // the row's words; its start position is the prefix of the chunk counts
 // before its first chunk, so the waves need no exchange.  A wave loads its
-// words 8 steps at a time, double-buffered (the next 8 steps' loads are in
+// words 16 steps at a time, double-buffered (the next 16 steps' loads are in
 // flight while the current ones are expanded), and writes them through its
 // own LDS staging area (dev::emit_step_staged), so every store instruction
 // is one contiguous run of columns.  Small workgroups let four rows share a
@@ -147,7 +147,7 @@ __global__ __launch_bounds__(EMIT_WG) void k_bitmap_emit(const uint4 *__restrict
                                                          const u64 *__restrict__ gpre, const u32 *__restrict__ list,
                                                          const u32 *__restrict__ perm, const u64 *__restrict__ off,
                                                          u32 words_per_row, u32 *__restrict__ cols) {
-    constexpr u32 NW = EMIT_WG / 64, BATCH = 8, SPAN = BATCH * 64;
+    constexpr u32 NW = EMIT_WG / 64, BATCH = 16, SPAN = BATCH * 64;
     __shared__ u32 stage[NW][dev::STAGE_WORDS];
     const u32 slot = list[blockIdx.x];
     const u32 r = perm[slot];
