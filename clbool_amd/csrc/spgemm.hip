@@ -133,16 +133,17 @@ __global__ void k_set_positions(const u32 *__restrict__ nnz_row, const u64 *__re
 }
 
 // Expands one dumped bitmap row (one workgroup per row) into its columns at
-// their final positions.  Each of the 4 waves owns a contiguous quarter of
+// their final positions.  Each of the 2 waves owns a contiguous half of
 // the row's words; its start position is the prefix of the chunk counts
 // before its first chunk, so the waves need no exchange.  A wave loads its
 // words 16 steps at a time, double-buffered (the next 16 steps' loads are in
 // flight while the current ones are expanded), and writes them through its
 // own LDS staging area (dev::emit_step_staged), so every store instruction
-// is one contiguous run of columns.  Small workgroups let four rows share a
-// CU, so one row's load latency overlaps another's expansion.
+// is one contiguous run of columns.  Two-wave workgroups (8.4 KB of staging
+// per wave) let nine rows share a CU, so one row's load latency overlaps
+// another's expansion (at 256 threads, four rows: 11% slower).
 // gpre = exclusive prefix of chunk_cnt over all slots.
-constexpr int EMIT_WG = 256;
+constexpr int EMIT_WG = 128;
 __global__ __launch_bounds__(EMIT_WG) void k_bitmap_emit(const uint4 *__restrict__ store, u32 nch,
                                                          const u64 *__restrict__ gpre, const u32 *__restrict__ list,
                                                          const u32 *__restrict__ perm, const u64 *__restrict__ off,
