@@ -69,12 +69,15 @@ class Pool {
     void *alloc(u64 bytes, u64 *cap_out);
     void release(void *p, u64 cap);
     void trim();
-    u64 cached_bytes() const { return cached_; }
+    u64 cached_bytes() const {
+        std::lock_guard<std::mutex> lk(mu_);
+        return cached_;
+    }
     int device() const { return device_; }
 
   private:
     int device_;
-    std::mutex mu_;
+    mutable std::mutex mu_;
     std::multimap<u64, void *> free_;  // capacity -> pointer
     u64 cached_ = 0;
 };
