@@ -57,6 +57,10 @@ def lib():
         L.orc_spgemm_gustavson.restype = C.POINTER(_OrcMat)
         L.orc_spgemm_gustavson.argtypes = [u32p, u32p, u32p, C.c_uint32, u32p, u32p, u32p, C.c_uint32,
                                            C.c_uint32, C.c_uint32, u32p, C.c_uint32, C.c_int]
+        L.orc_spgemm_digest.argtypes = [u32p, u32p, C.c_uint32, C.c_uint32, u32p, u32p, u32p, C.c_uint32,
+                                        C.c_uint32, C.c_int, u64p, u64p]
+        L.orc_dcsr_row_digest.restype = C.c_uint64
+        L.orc_dcsr_row_digest.argtypes = [u32p, u32p, C.c_uint64, C.c_uint32, C.c_uint32, C.c_int, u64p]
         L.orc_dense_bool_mm.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32]
         L.orc_workload.argtypes = [u32p, u32p, C.c_uint32, u32p, u32p, C.c_uint32, u64p]
         L.orc_coo_add.restype = C.POINTER(_OrcPairs)
@@ -217,6 +221,34 @@ def spgemm(a: Dcsr, b: Dcsr, row_sel=None, threads=None) -> Dcsr:
     ptr = lib().orc_spgemm_gustavson(_p32(ar), _p32(aw), _p32(ac), a.nzr, _p32(br), _p32(bw), _p32(bc), b.nzr,
                                      b.nrows, b.ncols, sp, ns, threads)
     return _take_mat(ptr, a.nrows, b.ncols)
+
+
+def spgemm_digest(a: Dcsr, b: Dcsr, r0=0, r1=None, threads=None):
+    """Per-row digests (nnz, sum of splitmix64(col)) of rows [r0, r1) of
+    A * B without materialising the product (checker for full-size runs)."""
+    if threads is None:
+        threads = min(16, os.cpu_count() or 1)
+    r1 = a.nzr if r1 is None else r1
+    cnt = np.zeros(max(0, r1 - r0), np.uint64)
+    h = np.zeros(max(0, r1 - r0), np.uint64)
+    ar, ac = _u32(a.rpt), _u32(a.cols)
+    br, bw, bc = _u32(b.rpt), _u32(b.rows), _u32(b.cols)
+    lib().orc_spgemm_digest(_p32(ar), _p32(ac), r0, r1, _p32(br), _p32(bw), _p32(bc), b.nzr, b.ncols, threads,
+                            cnt.ctypes.data_as(u64p), h.ctypes.data_as(u64p))
+    return cnt, h
+
+
+def dcsr_row_digest(rpt, cols, base=0, r0=0, r1=None, threads=None):
+    """Digests of stored rows [r0, r1) of a DCSR matrix from a chunk of its
+    columns starting at column position `base`; returns (hashes, rows whose
+    columns are not strictly ascending)."""
+    if threads is None:
+        threads = min(16, os.cpu_count() or 1)
+    rpt, cols = _u32(rpt), _u32(cols)
+    r1 = rpt.size - 1 if r1 is None else r1
+    h = np.zeros(max(0, r1 - r0), np.uint64)
+    bad = lib().orc_dcsr_row_digest(_p32(rpt), _p32(cols), base, r0, r1, threads, h.ctypes.data_as(u64p))
+    return h, int(bad)
 
 
 def dense_bool_mm(a, b):

@@ -362,6 +362,97 @@ orc_mat *orc_spgemm_gustavson(const uint32_t *a_rpt, const uint32_t *a_rows, con
     return m;
 }
 
+/* ------------------------------------------------------------------------ */
+/* Row digests (checker for products too large to compare column by column;
+ * no reference counterpart).  A row's digest is its column count and the
+ * sum, modulo 2^64, of splitmix64(col) over its columns (Vigna's splitmix64
+ * finaliser): equal column sets give equal digests in any order, and a
+ * changed, missing or extra column changes the sum unless two 64-bit mixes
+ * cancel.  Sortedness and uniqueness are checked separately on the GPU side
+ * (orc_dcsr_row_digest).                                                     */
+static inline uint64_t mix64(uint64_t x) {
+    uint64_t z = x + 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+/* Digests of the rows of C = A * B for stored rows [r0, r1) of A (Gustavson
+ * with a stamped dense marker, as orc_spgemm_gustavson, but nothing is
+ * materialised): cnt[i - r0] = nnz of C's row a_rows[i], hash[i - r0] its
+ * digest sum.  Rows with no output get (0, 0).                              */
+void orc_spgemm_digest(const uint32_t *a_rpt, const uint32_t *a_cols, uint32_t r0, uint32_t r1,
+                       const uint32_t *b_rpt, const uint32_t *b_rows, const uint32_t *b_cols, uint32_t b_nzr,
+                       uint32_t b_ncols, int threads, uint64_t *cnt, uint64_t *hash) {
+    if (r1 <= r0) return;
+    if (b_nzr == 0) {
+        memset(cnt, 0, (uint64_t)(r1 - r0) * sizeof(uint64_t));
+        memset(hash, 0, (uint64_t)(r1 - r0) * sizeof(uint64_t));
+        return;
+    }
+    uint32_t maxc = b_ncols ? b_ncols - 1 : 0;
+    for (uint64_t q = 0; q < b_rpt[b_nzr]; ++q) if (b_cols[q] > maxc) maxc = b_cols[q];
+    uint64_t width = (uint64_t)maxc + 1;
+    uint64_t map_n = (uint64_t)b_rows[b_nzr - 1] + 1;
+    uint32_t *dense_map = NULL;
+    if (map_n <= (1ull << 27)) {
+        dense_map = (uint32_t *)malloc(map_n * sizeof(uint32_t));
+        memset(dense_map, 0xFF, map_n * sizeof(uint32_t));
+        for (uint32_t q = 0; q < b_nzr; ++q) dense_map[b_rows[q]] = q;
+    }
+    if (threads <= 0) threads = 1;
+#ifdef _OPENMP
+#pragma omp parallel num_threads(threads)
+#endif
+    {
+        uint32_t *mark = (uint32_t *)calloc(width, sizeof(uint32_t));
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 16)
+#endif
+        for (int64_t i = (int64_t)r0; i < (int64_t)r1; ++i) {
+            uint32_t stamp = (uint32_t)(i - r0) + 1;
+            uint64_t c = 0, h = 0;
+            for (uint32_t j = a_rpt[i]; j < a_rpt[i + 1]; ++j) {
+                uint32_t pos = b_lookup(b_rows, b_nzr, dense_map, map_n, a_cols[j]);
+                if (pos == UINT32_MAX) continue;
+                for (uint32_t k = b_rpt[pos]; k < b_rpt[pos + 1]; ++k) {
+                    uint32_t col = b_cols[k];
+                    if (mark[col] != stamp) { mark[col] = stamp; ++c; h += mix64(col); }
+                }
+            }
+            cnt[i - r0] = c;
+            hash[i - r0] = h;
+        }
+        free(mark);
+    }
+    free(dense_map);
+}
+
+/* Digests of stored rows [r0, r1) of a DCSR matrix whose columns
+ * cols[rpt[r0] - base ...] are given (a chunk read back from the device):
+ * hash[r - r0] as above.  Returns the number of rows whose columns are not
+ * strictly ascending (0 for a canonical matrix).                             */
+uint64_t orc_dcsr_row_digest(const uint32_t *rpt, const uint32_t *cols, uint64_t base, uint32_t r0, uint32_t r1,
+                             int threads, uint64_t *hash) {
+    uint64_t bad = 0;
+    if (threads <= 0) threads = 1;
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(threads) schedule(dynamic, 64) reduction(+ : bad)
+#endif
+    for (int64_t r = (int64_t)r0; r < (int64_t)r1; ++r) {
+        uint64_t h = 0;
+        int ok = 1;
+        for (uint64_t q = rpt[r]; q < rpt[r + 1]; ++q) {
+            uint32_t c = cols[q - base];
+            if (q > rpt[r] && cols[q - 1 - base] >= c) ok = 0;
+            h += mix64(c);
+        }
+        hash[r - r0] = h;
+        bad += ok ? 0 : 1;
+    }
+    return bad;
+}
+
 void orc_dense_bool_mm(const uint8_t *a, const uint8_t *b, uint8_t *c, uint32_t m, uint32_t k, uint32_t n) {
     for (uint32_t i = 0; i < m; ++i)
         for (uint32_t j = 0; j < n; ++j) {

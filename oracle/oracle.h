@@ -24,7 +24,14 @@
  *   orc_transpose             dense transpose; transpose twice is the identity
  *   orc_submatrix             dense slicing of the brute-force matrix
  *   orc_reduce                dense row-any
- *   orc_mlp_*                 finite differences (gradients), closed-form Adam
+ *   orc_spgemm_digest,        a numpy restatement of splitmix64 over the
+ *   orc_dcsr_row_digest       Gustavson product's rows; a dropped or changed
+ *                             column and an unsorted row are caught
+ *   orc_mlp_forward           a numpy evaluation of relu(relu(x W1' + b1)
+ *                             W2' + b2) w3 + b3 with the documented layout (a
+ *                             planted missing ReLU is caught)
+ *   orc_mlp_loss_grad         finite differences of the loss
+ *   orc_adam_step             closed-form first step
  */
 #ifndef CLBOOL_ORACLE_H
 #define CLBOOL_ORACLE_H
@@ -89,6 +96,12 @@ orc_mat *orc_spgemm_gustavson(const uint32_t *a_rpt, const uint32_t *a_rows, con
 
 /* Dense brute force: C[i][j] = OR_k A[i][k] AND B[k][j] over byte masks
  * (row-major, a is m x k, b is k x n, c is m x n).  Pins the SpGEMM. */
+/* Row digests: count and sum of splitmix64(col) per row (checker only). */
+void orc_spgemm_digest(const uint32_t *a_rpt, const uint32_t *a_cols, uint32_t r0, uint32_t r1,
+                       const uint32_t *b_rpt, const uint32_t *b_rows, const uint32_t *b_cols, uint32_t b_nzr,
+                       uint32_t b_ncols, int threads, uint64_t *cnt, uint64_t *hash);
+uint64_t orc_dcsr_row_digest(const uint32_t *rpt, const uint32_t *cols, uint64_t base, uint32_t r0, uint32_t r1,
+                             int threads, uint64_t *hash);
 void orc_dense_bool_mm(const uint8_t *a, const uint8_t *b, uint8_t *c, uint32_t m, uint32_t k, uint32_t n);
 
 /* get_workload: src/coo/coo_utils.cpp:439-456 (and count_workload.cl:30-53):

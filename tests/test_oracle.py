@@ -310,3 +310,120 @@ def test_adam_first_step_closed_form():
     # first step: m_hat = g, v_hat = g^2, update = lr * g / (|g| + eps)
     expect = np.array([1.0 - 0.01, -2.0 + 0.01, 0.5], np.float32)
     assert np.allclose(p, expect, atol=1e-6)
+
+
+# ---- row digests ----------------------------------------------------------------
+def _splitmix64_np(x):
+    """splitmix64 finaliser (Vigna), restated in numpy uint64 arithmetic."""
+    with np.errstate(over="ignore"):
+        z = x.astype(np.uint64) + np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return z ^ (z >> np.uint64(31))
+
+
+def _row_hashes_np(m: oracle.Dcsr):
+    mixed = _splitmix64_np(m.cols)
+    rpt = m.rpt.astype(np.int64)
+    out = np.zeros(m.nzr, np.uint64)
+    for q in range(m.nzr):
+        with np.errstate(over="ignore"):
+            out[q] = np.sum(mixed[rpt[q]:rpt[q + 1]], dtype=np.uint64)
+    return out
+
+
+def test_splitmix64_known_values():
+    # splitmix64 as a generator from state 0: its first outputs are the
+    # finaliser applied to k * golden-gamma (published reference values)
+    g = np.uint64(0x9E3779B97F4A7C15)
+    with np.errstate(over="ignore"):
+        got = _splitmix64_np(np.array([0, 1], np.uint64) * g)
+    assert [int(x) for x in got] == [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4]
+
+
+def test_spgemm_digest_matches_product_rows():
+    rng = np.random.default_rng(21)
+    a = random_dense(rng, 90, 70, 0.06)
+    b = random_dense(rng, 70, 3000, 0.02)
+    A, B = dcsr_from_dense(a), dcsr_from_dense(b)
+    ref = dcsr_from_dense(oracle.dense_bool_mm(a, b))
+    cnt, h = oracle.spgemm_digest(A, B)
+    # stored rows of A with an empty product get (0, 0); the others match the
+    # brute-force product's rows in order
+    nz = cnt > 0
+    assert np.array_equal(A.rows[nz], ref.rows)
+    assert np.array_equal(cnt[nz].astype(np.int64), np.diff(ref.rpt.astype(np.int64)))
+    assert np.array_equal(h[nz], _row_hashes_np(ref))
+    assert np.all(h[~nz] == 0)
+    # a row range of A gives the matching slice
+    c2, h2 = oracle.spgemm_digest(A, B, 10, 40)
+    assert np.array_equal(c2, cnt[10:40]) and np.array_equal(h2, h[10:40])
+    # the GPU-side digest of the canonical product agrees, also from chunks
+    hd, bad = oracle.dcsr_row_digest(ref.rpt, ref.cols)
+    assert bad == 0 and np.array_equal(hd, h[nz])
+    k = int(ref.rpt[20])
+    hc, bad = oracle.dcsr_row_digest(ref.rpt, ref.cols[k:], base=k, r0=20, r1=ref.nzr)
+    assert bad == 0 and np.array_equal(hc, hd[20:])
+
+
+def test_digest_planted_mistakes_are_caught():
+    rng = np.random.default_rng(22)
+    a = random_dense(rng, 60, 60, 0.08)
+    A = dcsr_from_dense(a)
+    ref = dcsr_from_dense(oracle.dense_bool_mm(a, a))
+    h, _ = oracle.dcsr_row_digest(ref.rpt, ref.cols)
+    # a changed column (same count) changes that row's digest
+    cols = ref.cols.copy()
+    cols[ref.rpt[3]] ^= 1
+    h2, _ = oracle.dcsr_row_digest(ref.rpt, cols)
+    assert h2[3] != h[3] and np.array_equal(np.delete(h2, 3), np.delete(h, 3))
+    # two swapped columns keep the digest but are reported as unsorted
+    r = int(np.flatnonzero(np.diff(ref.rpt.astype(np.int64)) >= 2)[0])
+    cols = ref.cols.copy()
+    s = int(ref.rpt[r])
+    cols[s], cols[s + 1] = cols[s + 1], cols[s]
+    h3, bad = oracle.dcsr_row_digest(ref.rpt, cols)
+    assert bad == 1 and np.array_equal(h3, h)
+    # a product with a transposed operand has different digests
+    _, ht = oracle.spgemm_digest(A, dcsr_from_dense(np.ascontiguousarray(a.T)))
+    _, hp = oracle.spgemm_digest(A, A)
+    assert not np.array_equal(ht, hp)
+
+
+# ---- MLP forward against an independent evaluation -------------------------------
+def _mlp_numpy(p, x, F, H1, H2, relu1=True, relu2=True):
+    """relu(relu(x W1^T + b1) W2^T + b2) . w3 + b3, parameters laid out
+    W1[H1][F], b1[H1], W2[H2][H1], b2[H2], w3[H2], b3 (the layout that
+    include/clbool_c.h documents for CLB_MLP_PARAMS)."""
+    p = p.astype(np.float64)
+    o = np.cumsum([0, H1 * F, H1, H2 * H1, H2, H2])
+    W1, b1 = p[o[0]:o[1]].reshape(H1, F), p[o[1]:o[2]]
+    W2, b2 = p[o[2]:o[3]].reshape(H2, H1), p[o[3]:o[4]]
+    w3, b3 = p[o[4]:o[5]], p[o[5]]
+    h = x.astype(np.float64) @ W1.T + b1
+    h = np.maximum(h, 0) if relu1 else h
+    h = h @ W2.T + b2
+    h = np.maximum(h, 0) if relu2 else h
+    return h @ w3 + b3
+
+
+@pytest.mark.parametrize("F,H1,H2", [(5, 32, 32), (5, 8, 6), (3, 4, 7)])
+def test_mlp_forward_matches_numpy(F, H1, H2):
+    rng = np.random.default_rng(31 + H1)
+    n = oracle.mlp_param_count(F, H1, H2)
+    assert n == H1 * F + H1 + H2 * H1 + H2 + H2 + 1
+    p = (rng.standard_normal(n) * 0.7).astype(np.float32)
+    x = rng.standard_normal((200, F)).astype(np.float32)
+    y = oracle.mlp_forward(p, x, F, H1, H2).astype(np.float64)
+    ref = _mlp_numpy(p, x, F, H1, H2)
+    assert np.allclose(y, ref, rtol=1e-5, atol=1e-5)
+    # planted mistakes: either ReLU dropped, or W1 read transposed, must differ
+    for wrong in (_mlp_numpy(p, x, F, H1, H2, relu1=False), _mlp_numpy(p, x, F, H1, H2, relu2=False)):
+        assert np.max(np.abs(y - wrong)) > 1e-2
+    if F != H1:
+        q = p.copy()
+        q[:H1 * F] = p[:H1 * F].reshape(F, H1).T.reshape(-1)  # W1 read as [F][H1]
+        assert np.max(np.abs(y - _mlp_numpy(q, x, F, H1, H2))) > 1e-2
+    # the production parameter count
+    if (F, H1, H2) == (5, 32, 32):
+        assert n == 1281
