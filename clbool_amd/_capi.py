@@ -80,7 +80,7 @@ class ClosureOpts(C.Structure):
 
 class ClosureStats(C.Structure):
     _fields_ = [("products", C.c_uint64), ("added", C.c_uint64), ("changed", C.c_uint32),
-                ("spgemm_calls", C.c_uint32), ("ms", C.c_double)]
+                ("spgemm_calls", C.c_uint32), ("ms", C.c_double), ("max_block_bound", C.c_uint64)]
 
 
 CLOSURE_PER_TRIPLE = 0

@@ -111,15 +111,19 @@ __global__ __launch_bounds__(256) void k_row_bounds(const u32 *__restrict__ rpt,
     if (dev::lane() == 0) bound[r] = p < n ? p : n;
 }
 
-// cuts[b] = first stored row whose exclusive bound prefix is >= b * lim.
-__global__ void k_cuts(const u64 *__restrict__ pre, u32 nzr, u64 lim, u32 nb, u32 *__restrict__ cuts) {
+// cuts[b] = first stored row whose exclusive bound prefix is >= b * step
+// (pre holds nzr + 1 entries, pre[nzr] = the total).  Every row's bound is at
+// most n, so with step = lim - n + 1 a block [cuts[b], cuts[b+1]) has bound
+// pre[cuts[b+1]] - pre[cuts[b]] < (b+1)*step + n - b*step, i.e. at most lim.
+// bbound[b] receives that block bound.
+__global__ void k_cuts(const u64 *__restrict__ pre, u32 nzr, u64 step, u32 nb, u32 *__restrict__ cuts) {
     const u32 b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b > nb) return;
     if (b == nb) {
         cuts[b] = nzr;
         return;
     }
-    const u64 target = (u64)b * lim;
+    const u64 target = (u64)b * step;
     u32 lo = 0, hi = nzr;
     while (lo < hi) {
         const u32 mid = lo + (hi - lo) / 2;
@@ -127,6 +131,12 @@ __global__ void k_cuts(const u64 *__restrict__ pre, u32 nzr, u64 lim, u32 nb, u3
         else hi = mid;
     }
     cuts[b] = lo;
+}
+
+__global__ void k_block_bounds(const u64 *__restrict__ pre, const u32 *__restrict__ cuts, u32 nb,
+                               u64 *__restrict__ bbound) {
+    const u32 b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < nb) bbound[b] = pre[cuts[b + 1]] - pre[cuts[b]];
 }
 
 // For row blocks t = 0..nt of a product (row ids t*n + i): out[2t] = first
@@ -144,6 +154,15 @@ __global__ void k_row_splits(const u32 *__restrict__ rpt, const u32 *__restrict_
     }
     out[2 * t] = lo;
     out[2 * t + 1] = rpt[lo];
+}
+
+// Output bound of one stacked row block: the caller's block_bound (0 or
+// above 2^32 - 1: 2^32 - 1), raised to 2n when smaller (one row's bound is up
+// to n, and cuts need room for at least one row).
+static u64 closure_block_limit(u32 block_bound, u32 n) {
+    u64 lim = block_bound ? block_bound : 0xFFFFFFFFull;
+    if (lim < 2ull * n) lim = 2ull * n;
+    return lim < 0xFFFFFFFFull ? lim : 0xFFFFFFFFull;
 }
 
 // Builds the stack of the given matrices on the device.
@@ -262,20 +281,32 @@ clb_status clb_closure_step(clb_controls ctl, clb_dcsr *mats, uint32_t nmats, co
                     k_row_lengths<<<div_up(Z.nzr, 256), 256, 0, ctl->stream>>>(u32p(Z.rpt), u32p(Z.rows), Z.nzr,
                                                                                 zlen.ptr);
                     CLB_LAUNCHED("row_lengths");
-                    DevArray<u64> pre(ctl, Y.nzr), tot(ctl, 1);
+                    DevArray<u64> pre(ctl, (u64)Y.nzr + 1);
                     k_row_bounds<<<div_up(Y.nzr, 4), 256, 0, ctl->stream>>>(u32p(Y.rpt), u32p(Y.cols), Y.nzr,
                                                                                zlen.ptr, n, pre.ptr);
                     CLB_LAUNCHED("row_bounds");
-                    exclusive_scan_u64(ctl, pre.ptr, pre.ptr, Y.nzr, tot.ptr);
+                    exclusive_scan_u64(ctl, pre.ptr, pre.ptr, Y.nzr, pre.ptr + Y.nzr);
                     u64 total = 0;
-                    readback(ctl, &total, tot.ptr, 8);
-                    const u64 lim = o.block_bound && o.block_bound < 0xFFFFFFFFull ? o.block_bound : 0xFFFFFFFFull;
-                    const u32 nb = (u32)((total + lim - 1) / lim);
+                    readback(ctl, &total, pre.ptr + Y.nzr, 8);
+                    const u64 lim = closure_block_limit(o.block_bound, n);
+                    const u64 step = lim - n + 1;
+                    const u32 nb = (u32)((total + step - 1) / step);
                     DevArray<u32> cuts(ctl, (u64)nb + 1);
-                    k_cuts<<<div_up(nb + 1, 256), 256, 0, ctl->stream>>>(pre.ptr, Y.nzr, lim, nb, cuts.ptr);
+                    DevArray<u64> bb(ctl, nb ? nb : 1);
+                    k_cuts<<<div_up(nb + 1, 256), 256, 0, ctl->stream>>>(pre.ptr, Y.nzr, step, nb, cuts.ptr);
                     CLB_LAUNCHED("cuts");
+                    if (nb) {
+                        k_block_bounds<<<div_up(nb, 256), 256, 0, ctl->stream>>>(pre.ptr, cuts.ptr, nb, bb.ptr);
+                        CLB_LAUNCHED("block_bounds");
+                    }
                     std::vector<u32> hc(nb + 1);
+                    std::vector<u64> hb(nb);
                     readback(ctl, hc.data(), cuts.ptr, 4ull * (nb + 1));
+                    if (nb) readback(ctl, hb.data(), bb.ptr, 8ull * nb);
+                    for (u32 k = 0; k < nb; ++k) {
+                        if (hb[k] > lim) raise(RUN_KERNEL_FAILURE, "closure row block exceeds its output bound");
+                        if (hb[k] > st.max_block_bound) st.max_block_bound = hb[k];
+                    }
                     DevArray<u32> sp(ctl, 2ull * (ntriples + 1));
                     for (u32 k = 0; k < nb; ++k) {
                         if (hc[k + 1] <= hc[k]) continue;

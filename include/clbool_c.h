@@ -249,7 +249,9 @@ typedef struct clb_closure_opts {
     uint32_t mode;         /* CLB_CLOSURE_*                                   */
     uint32_t spgemm_mode;  /* CLB_SPGEMM_PLAIN or CLB_SPGEMM_LEARNED           */
     uint32_t block_bound;  /* batched: cut the stacked product into row blocks
-                              whose output bound stays below this (0: 2^32 - 1) */
+                              whose output bound (sum over the block's rows of
+                              min(products, n)) is at most this (0: 2^32 - 1;
+                              values below 2n are raised to 2n)           */
     uint32_t reserved;
 } clb_closure_opts;
 typedef struct clb_closure_stats {
@@ -258,6 +260,7 @@ typedef struct clb_closure_stats {
     uint32_t changed;      /* matrices that grew (0: fixed point reached)     */
     uint32_t spgemm_calls;
     double   ms;
+    uint64_t max_block_bound; /* batched: largest output bound of a row block  */
 } clb_closure_stats;
 /* One step X_t |= Y_t * Z_t for every triple t = (X, Y, Z) (indices into
  * mats, triples[3t..3t+2]); all products use the matrices as they were at

@@ -51,8 +51,10 @@ def test_closure_steps_match_oracle(controls, mode, block_bound, spgemm_mode):
             assert st["spgemm_calls"] == len(triples)
         elif block_bound == 0:
             assert st["spgemm_calls"] == 1
-        else:  # the stacked product is cut into several row blocks
+        else:  # the stacked product is cut into several row blocks, each
+            # within the bound (rows straddle multiples of the cut spacing)
             assert st["spgemm_calls"] > 1
+            assert 0 < st["max_block_bound"] <= block_bound
 
 
 def test_closure_reaches_fixed_point(controls):
