@@ -1,20 +1,24 @@
 """Benchmark of the boolean SpGEMM hot path on one MI355X.
 
-Workload (default): power-law A * A with A an R-MAT graph of scale 20, edge
+Workload (default): power-law A * A with A an R-MAT graph of scale 22, edge
 factor 16, (a, b, c, d) = (.57, .19, .19, .05), seed 1, duplicates and
-self-loops removed, uint32 DCSR.  Its product has about 9e9 nonzeros, more
-than uint32 indices hold, so a step multiplies A by A in row blocks whose
-outputs each fit (the same kernels; the blocks are independent).
+self-loops removed, uint32 DCSR; the same at scale 20 is reported beside it.
+Their products hold about 7e10 and 1e10 nonzeros, more than uint32 indices
+hold, so a step multiplies A by A in row blocks whose results each fit.
 
-A "step" is one full A * A over all row blocks with the inputs resident in
-HBM.  `value` is intermediate products per second of the whole step (the sum
-over rows of sum_{k in A_i} |B_k|, as count_workload.cl:47-51 counts them).
-The JSON line also carries the roofline of the dominant kernel (algorithmic
-bytes per launch over its HIP-event duration) and the CPU baseline (the
-reference test suite's host SpGEMM, matrix_multiplication_cpu restated in
-oracle/, timed on a bounded row sample).
+A "step" is one full A * A with the inputs resident in HBM, in each mode:
+plain (two passes), learned (one pass sized by the predictor) and bound (one
+pass sized by min(products, ncols), no predictor).  Each mode plans its own
+row blocks inside the step from what it knows (exact sizes from its symbolic
+pass, predicted capacities, or bounds; clb_spgemm_plan_*).  `value` is
+intermediate products per second of the learned step (the sum over rows of
+sum_{k in A_i} |B_k|, as count_workload.cl:47-51 counts them).  The JSON line
+also carries the roofline of the dominant kernel (algorithmic bytes per
+launch over its HIP-event duration) and the CPU baseline (the reference test
+suite's host SpGEMM, matrix_multiplication_cpu restated in oracle/, timed on
+a bounded row sample) beside a multi-threaded Gustavson on the host cores.
 
-    python bench.py [--gpus 1] [--steps 5] [--warmup 2] [--workload rmat20]
+    python bench.py [--gpus 1] [--steps 5] [--warmup 2] [--workload rmat22] [--also rmat20]
 """
 from __future__ import annotations
 
@@ -69,20 +73,6 @@ def row_blocks(prod, budget, ncols=None):
     return list(zip(bounds[:-1], bounds[1:]))
 
 
-def exact_row_sizes(cb, ctl, a_dev, a_rows, blocks):
-    """Output size of every stored row of A * A (untimed plain products over
-    `blocks`, each within uint32)."""
-    sizes = np.zeros(a_rows.size, np.int64)
-    for lo, hi in blocks:
-        C, _ = cb.dcsr.spgemm(ctl, a_dev, a_dev, mode=cb.SPGEMM_PLAIN, row_begin=lo, row_end=hi)
-        if C.nnz:
-            crow = C.rows_gpu.read(ctl, C.nzr)
-            crpt = C.rpt_gpu.read(ctl, C.nzr + 1).astype(np.int64)
-            sizes[np.searchsorted(a_rows, crow)] = np.diff(crpt)
-        del C
-    return sizes
-
-
 def validate_sample(cb, oracle, ctl, a_dev, A, blocks, mode, rng, rows=480):
     """Bit-exact check of sampled rows of every block against the oracle
     (about `rows` rows in all, at least 48 per block)."""
@@ -113,26 +103,56 @@ def validate_sample(cb, oracle, ctl, a_dev, A, blocks, mode, rng, rows=480):
     return checked
 
 
-def run_steps(cb, ctl, a_dev, blocks, mode, n, timing):
-    """n full A*A passes; returns (seconds, per-step stats of the last pass)."""
+def _new_agg():
+    return {"products": 0, "nnz": 0, "ms_kernel": [0.0] * 16, "launches": [0] * 16,
+            "bin_rows": [0] * 32, "bin_products": [0] * 32, "bin_a_nnz": [0] * 32, "bin_out_nnz": [0] * 32,
+            "fallback_rows": 0, "dense_rows": 0, "dense_nnz": 0, "blocks": 0, "replans": 0, "plan_ms": 0.0,
+            "err_rows": 0, "err_sum": 0.0, "capacity": 0}
+
+
+def _add_stats(agg, st):
+    agg["products"] += st["products"]
+    agg["nnz"] += st["nnz"]
+    agg["fallback_rows"] += st["fallback_rows"]
+    agg["dense_rows"] += st["dense_rows"]
+    agg["dense_nnz"] += st["dense_nnz"]
+    agg["capacity"] += st["capacity"]
+    agg["err_rows"] += st["nzr"]
+    agg["err_sum"] += st["pred_abs_log2_err"] * st["nzr"]
+    for k in ("ms_kernel", "launches", "bin_rows", "bin_products", "bin_a_nnz", "bin_out_nnz"):
+        agg[k] = [x + y for x, y in zip(agg[k], st[k])]
+
+
+def run_steps(cb, ctl, a_dev, mode, n, timing, block_bound):
+    """n full A*A products.  Each step plans its own row blocks (the mode's
+    sizing pass over all rows of A: symbolic counts, predicted capacities or
+    bounds) and then runs every block; the planning is inside the timed
+    region.  Returns (seconds, statistics of the last step)."""
     agg = None
     t0 = time.perf_counter()
     for _ in range(n):
-        agg = {"products": 0, "nnz": 0, "ms_kernel": [0.0] * 16, "launches": [0] * 16,
-               "bin_rows": [0] * 32, "bin_products": [0] * 32, "bin_a_nnz": [0] * 32, "bin_out_nnz": [0] * 32,
-               "fallback_rows": 0, "dense_rows": 0, "dense_nnz": 0}
-        for lo, hi in blocks:
-            C, st = cb.dcsr.spgemm(ctl, a_dev, a_dev, mode=mode, row_begin=lo, row_end=hi, timing=timing)
+        agg = _new_agg()
+        plan = cb.dcsr.plan(ctl, a_dev, a_dev, mode=mode, block_bound=block_bound)
+        agg["plan_ms"] = plan.info()["ms_plan"]
+        k = 0
+        while k < plan.nblocks:
+            C, st = plan.run(k, timing=timing)
             del C
-            agg["products"] += st["products"]
-            agg["nnz"] += st["nnz"]
-            agg["fallback_rows"] += st["fallback_rows"]
-            agg["dense_rows"] += st["dense_rows"]
-            agg["dense_nnz"] += st["dense_nnz"]
-            for k in ("ms_kernel", "launches", "bin_rows", "bin_products", "bin_a_nnz", "bin_out_nnz"):
-                agg[k] = [x + y for x, y in zip(agg[k], st[k])]
+            _add_stats(agg, st)
+            k += 1
+        info = plan.info()
+        agg["blocks"], agg["replans"] = info["nblocks"], info["replans"]
+        plan.close()
     ctl.sync()
     return time.perf_counter() - t0, agg
+
+
+def plan_blocks(cb, ctl, a_dev, mode, block_bound):
+    """The row blocks a plan of this mode cuts (for validation)."""
+    plan = cb.dcsr.plan(ctl, a_dev, a_dev, mode=mode, block_bound=block_bound)
+    blocks = [plan.block(k) for k in range(plan.nblocks)]
+    plan.close()
+    return blocks
 
 
 def train_predictor(cb, gen, ctl, log):
@@ -180,8 +200,9 @@ def _fingerprint(ctl, m):
 def bench_closure(cb, gen, ctl, max_iters, log, seed=0):
     """The 64-label closure on 2^18 vertices: 32 seeded triples, X |= Y * Z to
     a fixed point or max_iters steps.  Per-triple plain SpGEMMs (the
-    reference's call pattern) against one batched learned SpGEMM per step;
-    both runs start from the same matrices and must end with identical ones.
+    reference's call pattern), batched plain SpGEMMs (batching alone) and
+    batched learned SpGEMMs; all runs start from the same matrices and must
+    end with identical ones.
     Each mode starts from an emptied buffer pool and runs the whole closure
     once untimed, so both are timed with their own buffers cached."""
     n = 1 << 18
@@ -191,6 +212,7 @@ def bench_closure(cb, gen, ctl, max_iters, log, seed=0):
     out = {"labels": 64, "vertices": n, "triples": 32, "label_nnz": int(sum(d[2].size for d in dcsr))}
     finals = {}
     for name, mode, smode in (("per_triple_plain", cb.CLOSURE_PER_TRIPLE, cb.SPGEMM_PLAIN),
+                              ("batched_plain", cb.CLOSURE_BATCHED, cb.SPGEMM_PLAIN),
                               ("batched_learned", cb.CLOSURE_BATCHED, cb.SPGEMM_LEARNED)):
         ctl.trim()
         mats = [cb.matrix_dcsr_from_cpu(ctl, rpt, rows, cols, n) for rpt, rows, cols in dcsr]
@@ -216,8 +238,13 @@ def bench_closure(cb, gen, ctl, max_iters, log, seed=0):
                      "products_per_s": products / secs, "step_ms": [round(x, 2) for x in steps_ms],
                      "fixed_point": st["changed"] == 0, "final_nnz": int(sum(m.nnz for m in mats))}
         log(f"[bench] closure {name}: {iters} steps, {secs * 1e3:.1f} ms, {products / secs:.3e} products/s")
-    out["identical_results"] = finals["per_triple_plain"] == finals["batched_learned"]
-    out["speedup"] = out["per_triple_plain"]["seconds"] / out["batched_learned"]["seconds"]
+    out["identical_results"] = (finals["per_triple_plain"] == finals["batched_learned"] ==
+                                finals["batched_plain"])
+    sec = {k: out[k]["seconds"] for k in finals}
+    out["speedup"] = sec["per_triple_plain"] / sec["batched_learned"]  # learned + batched vs plain
+    out["batching_speedup"] = sec["per_triple_plain"] / sec["batched_plain"]  # same SpGEMM mode
+    out["learned_speedup_batched"] = sec["batched_plain"] / sec["batched_learned"]
+    out["fixed_point"] = out["batched_learned"]["fixed_point"]
     return out
 
 
@@ -419,25 +446,133 @@ def cpu_gustavson(oracle, A, threads, target_rows=20000, seed=11):
             "sample": f"{sel.size} sampled rows, {p} products in {dt:.2f} s (Gustavson, dense marker)"}
 
 
+MODE_CODES = {"plain": 0, "learned": 1, "bound": 2}
+
+
+def roofline_of(cb, agg, workload):
+    """Roofline of the dominant kernel family of a timed step: algorithmic
+    bytes of that family's launches over their HIP-event durations."""
+    fam_names = cb._capi.KERNEL_FAMILIES
+    fam = int(np.argmax(agg["ms_kernel"]))
+    launches = max(agg["launches"][fam], 1)
+    avg_ms = agg["ms_kernel"][fam] / launches
+    # A rows (rpt 8 B/row, a_cols 4 B + B row extent 8 B per A nonzero), 4 B
+    # per product read from B, 4 B per output column the family itself
+    # writes.  Dense bitmap rows are written by the emission family (4 B per
+    # column), not by the bitmap family that found them; the bitmap dump in
+    # between is not counted.  The compaction moves 4 B in and 4 B out per
+    # column it copies.
+    pass_, kind = divmod(fam, 4)
+    bins = {0: range(1, 2), 1: range(2, 10), 2: range(10, 32)}.get(kind, range(0))
+    writes = pass_ != 0
+    rows_f = sum(agg["bin_rows"][b] for b in bins)
+    bytes_total = sum(12 * agg["bin_a_nnz"][b] + 4 * agg["bin_products"][b] +
+                      (4 * agg["bin_out_nnz"][b] if writes else 0) for b in bins) + 8 * rows_f
+    if kind == 2 and writes:
+        bytes_total -= 4 * agg["dense_nnz"]
+    if fam_names[fam] == "onepass_compact":
+        bytes_total = 8 * (agg["nnz"] - agg["dense_nnz"])
+    elif kind == 3:
+        bytes_total = 4 * agg["dense_nnz"]
+    bytes_per_launch = bytes_total / launches
+    achieved = bytes_per_launch / (avg_ms / 1e3) / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
+    if os.path.exists(tpath):
+        try:
+            traffic = json.load(open(tpath)).get(fam_names[fam])
+        except Exception:
+            traffic = None
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "kernel": fam_names[fam], "launches": launches, "avg_launch_ms": round(avg_ms, 4),
+            "algorithmic_bytes_per_launch": int(bytes_per_launch)}
+
+
+def step_algorithmic_bytes(a_nzr, a_nnz, products, c_nzr_est, nnz):
+    """Compulsory bytes of one A*B over all blocks (single pass over every
+    operand and the output)."""
+    return 4 * (2 * a_nzr + 1) + 12 * a_nnz + 4 * products + 4 * (2 * c_nzr_est + 1) + 4 * nnz
+
+
+def bench_product(cb, gen, ctl, oracle, workload, modes, steps, warmup, tuned, validate, block_bound, log):
+    """A * A of an R-MAT workload, every mode planning its own blocks inside
+    the timed step."""
+    from clbool_amd import autotune as at
+    scale, ef, seed = WORKLOADS[workload]
+    t0 = time.perf_counter()
+    r, c = gen.rmat(scale, ef, seed)
+    rpt, rows, cols = gen.to_dcsr(r, c)
+    n = 1 << scale
+    del r, c
+    log(f"[bench] {workload}: R-MAT scale {scale} ef {ef} seed {seed} generated in {time.perf_counter() - t0:.1f} s")
+    a_dev = cb.matrix_dcsr_from_cpu(ctl, rpt, rows, cols, n, n)
+    prod = row_products(rpt, rows, cols, n, rpt, rows)
+    total_products = int(prod.sum())
+    A = oracle.Dcsr(rpt, rows, cols, n, n)
+    config = {"workload": f"rmat{scale}_ef{ef}_AxA", "scale": scale, "edge_factor": ef, "seed": seed,
+              "rmat_abcd": [0.57, 0.19, 0.19, 0.05], "a_nnz": int(cols.size), "a_nzr": int(rows.size),
+              "products": total_products, "block_bound": int(block_bound)}
+    results, validated = {}, {}
+    for m in modes:
+        at.apply(ctl, tuned[m])
+        if validate:
+            blocks = plan_blocks(cb, ctl, a_dev, MODE_CODES[m], block_bound)
+            validated[m] = validate_sample(cb, oracle, ctl, a_dev, A, blocks, MODE_CODES[m],
+                                           np.random.default_rng(3))
+            log(f"[bench] {workload} {m}: {validated[m]} sampled rows bit-exact vs oracle")
+        ctl.trim()
+        run_steps(cb, ctl, a_dev, MODE_CODES[m], warmup, False, block_bound)
+        secs, agg = run_steps(cb, ctl, a_dev, MODE_CODES[m], steps, True, block_bound)
+        ms = 1e3 * secs / steps
+        results[m] = {"ms_per_step": ms, "products_per_s": agg["products"] / (ms / 1e3),
+                      "nnz_per_s": agg["nnz"] / (ms / 1e3), "nnz": agg["nnz"], "agg": agg}
+        log(f"[bench] {workload} {m}: {ms:.2f} ms/step ({agg['blocks']} blocks, plan {agg['plan_ms']:.1f} ms), "
+            f"{results[m]['products_per_s']:.3e} products/s, fallback rows {agg['fallback_rows']}")
+        assert agg["products"] == total_products
+    nnzs = {m: v["nnz"] for m, v in results.items()}
+    assert len(set(nnzs.values())) == 1, f"modes disagree on nnz: {nnzs}"
+    fam_names = cb._capi.KERNEL_FAMILIES
+    out = {"config": config, "validated_rows": validated, "modes": {}}
+    for m, v in results.items():
+        agg = v["agg"]
+        out["modes"][m] = {
+            "ms_per_step": v["ms_per_step"], "products_per_s": v["products_per_s"], "nnz_per_s": v["nnz_per_s"],
+            "row_blocks": agg["blocks"], "replans": agg["replans"], "plan_ms": agg["plan_ms"],
+            "ms_kernel": dict(zip(fam_names, agg["ms_kernel"])), "launches": dict(zip(fam_names, agg["launches"])),
+            "fallback_rows": agg["fallback_rows"], "fallback_rate": agg["fallback_rows"] / max(1, int(rows.size)),
+            "pred_abs_log2_err": agg["err_sum"] / agg["err_rows"] if m == "learned" and agg["err_rows"] else None,
+            "capacity_over_nnz": agg["capacity"] / max(1, agg["nnz"]) if m != "plain" else None,
+            "dense_rows": agg["dense_rows"], "dense_nnz": agg["dense_nnz"], "tuning": tuned[m],
+            "roofline": roofline_of(cb, agg, workload),
+            "step_hbm_frac": step_algorithmic_bytes(int(rows.size), int(cols.size), total_products,
+                                                    int(rows.size), agg["nnz"]) / (v["ms_per_step"] / 1e3) / 1e9
+            / HBM_PEAK_GBS}
+    out["nnz_out"] = int(next(iter(nnzs.values())))
+    for x, y in (("learned", "plain"), ("bound", "plain"), ("learned", "bound")):
+        if x in results and y in results:
+            out[f"{x}_vs_{y}"] = results[x]["products_per_s"] / results[y]["products_per_s"]
+    return out, A
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="rmat20", choices=sorted(WORKLOADS))
-    ap.add_argument("--mode", default="auto", choices=["auto", "plain", "learned"])
+    ap.add_argument("--workload", default="rmat22", choices=sorted(WORKLOADS))
+    ap.add_argument("--also", default="rmat20", help="comma-separated workloads reported beside the main one "
+                    "('' for none)")
+    ap.add_argument("--mode", default="all", choices=["all", "plain", "learned", "bound"])
     ap.add_argument("--block-bound", type=float, default=float((1 << 32) - 1),
-                    help="largest output of one row block: its exact output size, or with --bound-blocks its "
-                         "output bound (sum of min(products, ncols) over its rows)")
-    ap.add_argument("--bound-blocks", action="store_true",
-                    help="cut row blocks by the output bound, without the exact sizing pass")
+                    help="largest result of one row block (nonzeros)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-validate", action="store_true")
-    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--closure-iters", type=int, default=4,
                     help="steps of the 64-label closure workload reported beside the main metric (0: skip)")
     ap.add_argument("--tuning-file", default=None,
-                    help="JSON written by a previous run (--save-tuning): use its knobs instead of tuning")
+                    help="JSON of knobs per mode (default profiles/tuning_<workload>.json when it exists)")
+    ap.add_argument("--autotune", action="store_true", help="tune the knobs on the workload first")
     ap.add_argument("--save-tuning", default=None)
     ap.add_argument("--hypersparse", type=int, default=0,
                     help="also run the hypersparse learned-sizing workload, training on this many instances")
@@ -482,127 +617,64 @@ def main():
         sys.exit(2)
 
     import clbool_amd as cb
+    from clbool_amd import autotune as at
     from clbool_amd import generators as gen
+    import oracle  # checker only: validates sampled rows and times the CPU baseline
 
-    scale, ef, seed = WORKLOADS[args.workload]
-    log(f"[bench] generating R-MAT scale {scale} ef {ef} seed {seed}")
-    r, c = gen.rmat(scale, ef, seed)
-    rpt, rows, cols = gen.to_dcsr(r, c)
-    n = 1 << scale
-    del r, c
     ctl = cb.create_controls()
     dev = cb.show_devices()[0]
-    a_dev = cb.matrix_dcsr_from_cpu(ctl, rpt, rows, cols, n, n)
-    prod = row_products(rpt, rows, cols, n, rpt, rows)
-    blocks = row_blocks(prod, int(args.block_bound), ncols=n)
-    if not args.bound_blocks:
-        # the output can be cut closer: exact row sizes from one untimed plain
-        # product over the bound blocks, then blocks of at most 2^32 - 1
-        # output nonzeros (fewer blocks, less fixed work per step)
-        sizes = exact_row_sizes(cb, ctl, a_dev, rows, blocks)
-        blocks = row_blocks(sizes, min(int(args.block_bound), (1 << 32) - 1))
-    total_products = int(prod.sum())
-    log(f"[bench] A nnz={cols.size} nzr={rows.size} products={total_products} blocks={len(blocks)}")
-
-    modes = ["plain"] if args.mode == "plain" else (["learned"] if args.mode == "learned" else ["plain", "learned"])
-    code = {"plain": cb.SPGEMM_PLAIN, "learned": cb.SPGEMM_LEARNED}
+    modes = ["plain", "learned", "bound"] if args.mode == "all" else [args.mode]
     train = train_predictor(cb, gen, ctl, log) if "learned" in modes else None
-
-    from clbool_amd import autotune as at
     base_tuning = at.tuning_dict(ctl.tuning())
-    tuned = {}
-    loaded = json.load(open(args.tuning_file)) if args.tuning_file else {}
-    for m in modes:
-        at.apply(ctl, base_tuning)
-        if m in loaded:
-            tuned[m] = loaded[m]
-            log(f"[bench] {m}: tuning from {args.tuning_file}: {tuned[m]}")
-        elif not args.no_autotune:
-            sample = blocks
-            res = at.autotune(ctl, a_dev, a_dev, sample, mode=code[m], repeats=2, log=log)
+    tpath = args.tuning_file or os.path.join(ROOT, "profiles", f"tuning_{args.workload}.json")
+    loaded = json.load(open(tpath)) if os.path.exists(tpath) else {}
+    tuned = {m: {**base_tuning, **loaded.get(m, {})} for m in modes}
+    if loaded:
+        log(f"[bench] knobs from {os.path.relpath(tpath, ROOT)}")
+    block_bound = int(args.block_bound)
+
+    if args.autotune:
+        scale, ef, seed = WORKLOADS[args.workload]
+        r, c = gen.rmat(scale, ef, seed)
+        rpt, rows, cols = gen.to_dcsr(r, c)
+        a_t = cb.matrix_dcsr_from_cpu(ctl, rpt, rows, cols, 1 << scale, 1 << scale)
+        for m in modes:
+            at.apply(ctl, tuned[m])
+            blocks = plan_blocks(cb, ctl, a_t, MODE_CODES[m], block_bound)
+            res = at.autotune(ctl, a_t, a_t, blocks, mode=MODE_CODES[m], repeats=2, log=log)
             tuned[m] = res["tuning"]
             log(f"[bench] {m}: autotuned {res['tuning']} over {res['evaluated']} settings")
-        else:
-            tuned[m] = base_tuning
+        del a_t
+        if args.save_tuning:
+            with open(args.save_tuning, "w") as f:
+                json.dump(tuned, f)
 
-    if args.save_tuning:
-        with open(args.save_tuning, "w") as f:
-            json.dump(tuned, f)
-
-    import oracle  # checker only: validates sampled rows and times the CPU baseline
-    A = oracle.Dcsr(rpt, rows, cols, n, n)
-    validated = {}
-    results = {}
-    for m in modes:
-        at.apply(ctl, tuned[m])
-        if not args.no_validate:
-            validated[m] = validate_sample(cb, oracle, ctl, a_dev, A, blocks, code[m], np.random.default_rng(3))
-            log(f"[bench] {m}: {validated[m]} sampled rows bit-exact vs oracle")
-        run_steps(cb, ctl, a_dev, blocks, code[m], args.warmup, False)
-        secs, agg = run_steps(cb, ctl, a_dev, blocks, code[m], args.steps, True)
-        ms = 1e3 * secs / args.steps
-        results[m] = {"ms_per_step": ms, "products_per_s": agg["products"] / (ms / 1e3),
-                      "nnz_per_s": agg["nnz"] / (ms / 1e3), "nnz": agg["nnz"], "agg": agg}
-        log(f"[bench] {m}: {ms:.2f} ms/step, {results[m]['products_per_s']:.3e} products/s, "
-            f"fallback rows {agg['fallback_rows']}")
-
-    best = "learned" if "learned" in results else "plain"
-    R = results[best]
-    agg = R["agg"]
-    # dominant kernel family over the timed region: pass x kind (see
-    # CLB_KF in include/clbool_c.h)
-    fam_names = cb._capi.KERNEL_FAMILIES
-    fam = int(np.argmax(agg["ms_kernel"]))
-    launches = max(agg["launches"][fam], 1)
-    avg_ms = agg["ms_kernel"][fam] / launches
-    # algorithmic bytes of that family's launches: A rows (rpt 8 B/row,
-    # a_cols 4 B + B row extent 8 B per A nonzero), 4 B per product read from
-    # B, 4 B per output column the family itself writes.  Dense bitmap rows
-    # are written by the emission family (4 B per column), not by the bitmap
-    # family that found them; the bitmap dump in between is not counted.
-    pass_, kind = divmod(fam, 4)
-    bins = {0: range(1, 2), 1: range(2, 10), 2: range(10, 32)}.get(kind, range(0))
-    writes = pass_ != 0
-    rows_f = sum(agg["bin_rows"][b] for b in bins)
-    bytes_total = sum(12 * agg["bin_a_nnz"][b] + 4 * agg["bin_products"][b] +
-                      (4 * agg["bin_out_nnz"][b] if writes else 0) for b in bins) + 8 * rows_f
-    if kind == 2 and writes:
-        bytes_total -= 4 * agg["dense_nnz"]
-    if kind == 3:
-        bytes_total = 4 * agg["dense_nnz"]
-    bytes_per_launch = bytes_total / launches
-    achieved = bytes_per_launch / (avg_ms / 1e3) / 1e9
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json")
-    if os.path.exists(tpath):
-        try:
-            traffic = json.load(open(tpath)).get(fam_names[fam])
-        except Exception:
-            traffic = None
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": fam_names[fam], "launches": launches, "avg_launch_ms": round(avg_ms, 4),
-                "algorithmic_bytes_per_launch": int(bytes_per_launch)}
-
+    main_res, A = bench_product(cb, gen, ctl, oracle, args.workload, modes, args.steps, args.warmup, tuned,
+                                not args.no_validate, block_bound, log)
+    best = "learned" if "learned" in main_res["modes"] else modes[0]
+    R = main_res["modes"][best]
     out = {
         "metric": METRIC, "value": R["products_per_s"], "unit": "products/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": R["ms_per_step"], "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "u32", "data": "synthetic",
-        "config": {"workload": f"rmat{scale}_ef{ef}_AxA", "scale": scale, "edge_factor": ef, "seed": seed,
-                   "rmat_abcd": [0.57, 0.19, 0.19, 0.05], "a_nnz": int(cols.size), "a_nzr": int(rows.size),
-                   "products": total_products, "nnz_out": int(R["nnz"]), "row_blocks": len(blocks),
+        "config": {**main_res["config"], "nnz_out": main_res["nnz_out"], "row_blocks": R["row_blocks"],
                    "mode": best, "device": dev["arch"], "compute_units": dev["compute_units"]},
         "output_nnz_per_s": R["nnz_per_s"],
-        "modes": {m: {"ms_per_step": v["ms_per_step"], "products_per_s": v["products_per_s"],
-                      "ms_kernel": dict(zip(fam_names, v["agg"]["ms_kernel"])),
-                      "fallback_rows": v["agg"]["fallback_rows"], "dense_rows": v["agg"]["dense_rows"],
-                      "dense_nnz": v["agg"]["dense_nnz"], "tuning": tuned[m]} for m, v in results.items()},
+        "roofline": R["roofline"],
         "predictor_training": train,
-        "validated_rows": validated,
-        "roofline": roofline,
+        "pred_abs_log2_err": R.get("pred_abs_log2_err"),
+        "fallback_rate": R.get("fallback_rate"),
     }
-    if "learned" in results and "plain" in results:
-        out["learned_vs_plain"] = results["learned"]["products_per_s"] / results["plain"]["products_per_s"]
+    for k in ("learned_vs_plain", "bound_vs_plain", "learned_vs_bound", "modes", "validated_rows"):
+        if k in main_res:
+            out[k] = main_res[k]
+    for w in [w for w in args.also.split(",") if w and w != args.workload]:
+        wt = os.path.join(ROOT, "profiles", f"tuning_{w}.json")
+        wl = json.load(open(wt)) if os.path.exists(wt) else {}
+        tw = {m: {**base_tuning, **wl.get(m, {})} for m in modes}
+        res, _ = bench_product(cb, gen, ctl, oracle, w, modes, args.steps, args.warmup, tw, not args.no_validate,
+                               block_bound, log)
+        out[w] = res
     if args.closure_iters > 0 and "learned" in modes:
         at.apply(ctl, tuned.get("learned", base_tuning))
         out["closure"] = bench_closure(cb, gen, ctl, args.closure_iters, log)
@@ -611,7 +683,15 @@ def main():
     if not args.no_cpu_baseline:
         log("[bench] timing the CPU baseline")
         out["cpu_baseline"] = cpu_baseline(oracle, A)
-        out["cpu_gustavson"] = cpu_gustavson(oracle, A, min(16, os.cpu_count() or 1))
+        try:
+            nproc = len(os.sched_getaffinity(0))
+        except AttributeError:
+            nproc = os.cpu_count() or 1
+        threads = int(os.environ.get("OMP_NUM_THREADS", nproc))
+        out["cpu_gustavson"] = cpu_gustavson(oracle, A, threads)
+        out["host"] = {"nproc": nproc, "os_cpu_count": os.cpu_count(), "omp_num_threads": threads}
+        out["clbool_cpu_opencl"] = ("not run: clBool's create_controls asks only for GPU devices "
+                                    "(src/common/env.cpp:21) and no CPU OpenCL runtime is installed")
     print(json.dumps(out), flush=True)
 
 

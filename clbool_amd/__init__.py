@@ -29,6 +29,7 @@ STATUS_NAMES = [
 ]
 SPGEMM_PLAIN = _capi.SPGEMM_PLAIN
 SPGEMM_LEARNED = _capi.SPGEMM_LEARNED
+SPGEMM_BOUND = _capi.SPGEMM_BOUND
 
 
 class ClboolError(RuntimeError):
@@ -360,6 +361,59 @@ def _spgemm(controls, a: matrix_dcsr, b: matrix_dcsr, mode=SPGEMM_PLAIN, row_beg
     return matrix_dcsr._from_c(m), st.as_dict()
 
 
+class SpgemmPlan:
+    """A * B taken in row blocks planned by the library (clb_spgemm_plan_*):
+    the mode's sizing pass over all of A's stored rows, then consecutive
+    blocks whose results hold at most block_bound nonzeros.  Iterating yields
+    (C_block, stats) for every block; a learned block that exceeds the bound
+    is cut again on the fly, so the block count can grow while iterating."""
+
+    def __init__(self, controls, a: matrix_dcsr, b: matrix_dcsr, mode=SPGEMM_PLAIN, block_bound=0):
+        self.controls = controls
+        self.handle = C.c_void_p()
+        ac, bc = a._c(), b._c()
+        _check(lib().clb_spgemm_plan_create(controls.handle, C.byref(ac), C.byref(bc), mode, int(block_bound),
+                                            C.byref(self.handle)))
+
+    def info(self) -> dict:
+        i = _capi.PlanInfo()
+        _check(lib().clb_spgemm_plan_info_get(self.handle, C.byref(i)))
+        return {f: getattr(i, f) for f, _ in i._fields_ if f != "reserved"}
+
+    @property
+    def nblocks(self) -> int:
+        return self.info()["nblocks"]
+
+    def block(self, k: int):
+        rb, re = C.c_uint32(), C.c_uint32()
+        _check(lib().clb_spgemm_plan_block(self.handle, k, C.byref(rb), C.byref(re)))
+        return rb.value, re.value
+
+    def run(self, k: int, timing=False):
+        m = _Dcsr()
+        st = SpgemmStats()
+        _check(lib().clb_spgemm_plan_run(self.controls.handle, self.handle, k, int(timing), C.byref(m),
+                                         C.byref(st)))
+        return matrix_dcsr._from_c(m), st.as_dict()
+
+    def __iter__(self):
+        k = 0
+        while k < self.nblocks:
+            yield self.run(k)
+            k += 1
+
+    def close(self):
+        if self.handle:
+            lib().clb_spgemm_plan_release(self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def _kronecker_product(controls, a: matrix_dcsr, b: matrix_dcsr) -> matrix_dcsr:
     """dcsr::kronecker_product (dcsr_kronecker_product.cpp:6-65)."""
     m, ac, bc = _Dcsr(), a._c(), b._c()
@@ -388,6 +442,13 @@ def _reduce(controls, a: matrix_dcsr) -> matrix_dcsr:
     return matrix_dcsr._from_c(m)
 
 
+def _digest(controls, a: matrix_dcsr) -> int:
+    """64-bit device-side digest of a DCSR matrix (clb_dcsr_digest)."""
+    h, ac = C.c_uint64(), a._c()
+    _check(lib().clb_dcsr_digest(controls.handle, C.byref(ac), C.byref(h)))
+    return h.value
+
+
 def _dcsr_addition(controls, a: matrix_dcsr, b: matrix_dcsr) -> matrix_dcsr:
     m, ac, bc = _Dcsr(), a._c(), b._c()
     _check(lib().clb_dcsr_addition(controls.handle, C.byref(ac), C.byref(bc), C.byref(m)))
@@ -414,7 +475,8 @@ def _csr_matrix_addition(controls, a: matrix_csr, b: matrix_csr) -> matrix_csr:
 
 
 dcsr = types.SimpleNamespace(matrix_multiplication_hash=_matrix_multiplication_hash,
-                             kronecker_product=_kronecker_product, spgemm=_spgemm, addition=_dcsr_addition,
+                             kronecker_product=_kronecker_product, spgemm=_spgemm, plan=SpgemmPlan,
+                             addition=_dcsr_addition, digest=_digest,
                              transpose=_transpose, submatrix=_submatrix, reduce=_reduce)
 coo = types.SimpleNamespace(matrix_addition=_coo_matrix_addition, sort_arrays=_coo_sort_arrays)
 csr = types.SimpleNamespace(matrix_addition=_csr_matrix_addition)

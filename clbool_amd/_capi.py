@@ -18,6 +18,8 @@ ctl_t = C.c_void_p
 CLB_MAX_BINS = 32
 SPGEMM_PLAIN = 0
 SPGEMM_LEARNED = 1
+SPGEMM_BOUND = 2
+TUNE_BITMAP_PRED_CAPS = 0  # index into Tuning.reserved
 
 
 class Dcsr(C.Structure):
@@ -71,6 +73,13 @@ class SpgemmStats(C.Structure):
 KERNEL_PASSES = ["sym", "num", "onepass", "fallback"]
 KERNEL_KINDS = ["tiny", "hash", "bitmap", "emit"]
 KERNEL_FAMILIES = [f"{p}_{k}" for p in KERNEL_PASSES for k in KERNEL_KINDS]
+# the re-run has no emission launch; its slot holds the one pass's compaction
+KERNEL_FAMILIES[15] = "onepass_compact"
+
+
+class PlanInfo(C.Structure):
+    _fields_ = [("mode", C.c_uint32), ("nblocks", C.c_uint32), ("replans", C.c_uint32), ("reserved", C.c_uint32),
+                ("products", C.c_uint64), ("planned_nnz", C.c_uint64), ("ms_plan", C.c_double)]
 
 
 class ClosureOpts(C.Structure):
@@ -144,11 +153,19 @@ SIGNATURES = [
     ("clb_dcsr_spgemm_hash", C.c_int32, [ctl_t, C.POINTER(Dcsr), C.POINTER(Dcsr), C.POINTER(Dcsr)]),
     ("clb_dcsr_spgemm", C.c_int32, [ctl_t, C.POINTER(Dcsr), C.POINTER(Dcsr), C.POINTER(SpgemmOpts),
                                     C.POINTER(Dcsr), C.POINTER(SpgemmStats)]),
+    ("clb_spgemm_plan_create", C.c_int32, [ctl_t, C.POINTER(Dcsr), C.POINTER(Dcsr), C.c_uint32, C.c_uint64,
+                                           C.POINTER(C.c_void_p)]),
+    ("clb_spgemm_plan_info_get", C.c_int32, [C.c_void_p, C.POINTER(PlanInfo)]),
+    ("clb_spgemm_plan_block", C.c_int32, [C.c_void_p, C.c_uint32, u32p, u32p]),
+    ("clb_spgemm_plan_run", C.c_int32, [ctl_t, C.c_void_p, C.c_uint32, C.c_uint32, C.POINTER(Dcsr),
+                                        C.POINTER(SpgemmStats)]),
+    ("clb_spgemm_plan_release", C.c_int32, [C.c_void_p]),
     ("clb_dcsr_kronecker", C.c_int32, [ctl_t, C.POINTER(Dcsr), C.POINTER(Dcsr), C.POINTER(Dcsr)]),
     ("clb_dcsr_transpose", C.c_int32, [ctl_t, C.POINTER(Dcsr), C.POINTER(Dcsr)]),
     ("clb_dcsr_submatrix", C.c_int32, [ctl_t, C.POINTER(Dcsr), C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
                                        C.POINTER(Dcsr)]),
     ("clb_dcsr_reduce", C.c_int32, [ctl_t, C.POINTER(Dcsr), C.POINTER(Dcsr)]),
+    ("clb_dcsr_digest", C.c_int32, [ctl_t, C.POINTER(Dcsr), C.POINTER(C.c_uint64)]),
     ("clb_coo_addition", C.c_int32, [ctl_t, C.POINTER(Coo), C.POINTER(Coo), C.POINTER(Coo)]),
     ("clb_csr_addition", C.c_int32, [ctl_t, C.POINTER(Csr), C.POINTER(Csr), C.POINTER(Csr)]),
     ("clb_dcsr_addition", C.c_int32, [ctl_t, C.POINTER(Dcsr), C.POINTER(Dcsr), C.POINTER(Dcsr)]),

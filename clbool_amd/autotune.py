@@ -3,8 +3,11 @@
 Replaces the hard-coded thresholds of clBool's hash SpGEMM
 (src/dcsr/dcsr_matrix_multiplication_hash.cpp:13-47).  The tuner times the
 library on a sample of the caller's own workload (row blocks of A * B) and
-walks the knobs one at a time (coordinate descent), keeping a setting only if
-it is faster and gives the same result (nnz and a checksum of sampled rows).
+walks the knobs one at a time (coordinate descent).  A setting is kept only
+if every block's result is the same as with the starting knobs (sizes and the
+device-side digest of the whole matrix, clb_dcsr_digest) and its best time
+over the repeats beats the current best by at least `min_gain` (2% by
+default: smaller differences are within run-to-run noise).
 The winning knobs are stored per device (architecture and CU count) in a JSON
 file and can be reloaded without tuning again.
 """
@@ -25,16 +28,21 @@ KNOBS = {
     "bitmap_slot_ratio": [1, 2, 4],
     "tiny_max_products": [16, 32, 64],
     "learned_margin_pct": [10, 25, 50],
+    "bitmap_pred_caps": [0, 1],   # one pass: bitmap rows sized by the prediction (1) or the bound (0)
+    "gather_combine": [0, 1],     # bitmap gather: one LDS atomic per word and wave (1) or per product (0)
 }
+# knobs that live in clb_tuning.reserved[]
+RESERVED = {"bitmap_pred_caps": 0, "gather_combine": 1}
+LEARNED_ONLY = ("learned_margin_pct", "bitmap_pred_caps")
 
 
 def _get(t, k):
-    return t.reserved[int(k[8:])] if k.startswith("reserved") else getattr(t, k)
+    return t.reserved[RESERVED[k]] if k in RESERVED else getattr(t, k)
 
 
 def _set(t, k, v):
-    if k.startswith("reserved"):
-        t.reserved[int(k[8:])] = v
+    if k in RESERVED:
+        t.reserved[RESERVED[k]] = v
     else:
         setattr(t, k, v)
 
@@ -52,20 +60,27 @@ def apply(controls, d: dict):
 
 
 def _checksum(cb, controls, c) -> tuple:
+    """(nnz, nzr, digest of the whole result): a setting that changes any
+    entry of any row changes the digest."""
     if c.nnz == 0:
-        return (0, 0)
-    k = min(c.nnz, 1 << 16)
-    head = c.cols_gpu.read(controls, k)
-    tail = c.cols_gpu.read(controls, k, offset=4 * (c.nnz - k))
-    return (int(c.nnz), int(head.astype(np.uint64).sum() ^ (tail.astype(np.uint64).sum() << np.uint64(1))))
+        return (0, 0, 0)
+    return (int(c.nnz), int(c.nzr), cb.dcsr.digest(controls, c))
 
 
-def autotune(controls, a, b, blocks, mode=0, repeats=2, knobs=None, log=None, passes=2) -> dict:
+def accept(t_new, t_cur, sums_new, sums_ref, min_gain=0.02, what=""):
+    """Decision of one trial: raises if the result differs from the
+    reference; True if the trial is at least min_gain faster."""
+    if sums_new != sums_ref:
+        raise AssertionError(f"tuning {what} changed the result")
+    return t_new < (1.0 - min_gain) * t_cur
+
+
+def autotune(controls, a, b, blocks, mode=0, repeats=3, knobs=None, log=None, passes=2, min_gain=0.02) -> dict:
     """Coordinate descent over KNOBS on the given row blocks of A * B (the
     knobs interact, so the sweep over all of them runs `passes` times)."""
     import clbool_amd as cb
 
-    knobs = knobs or [k for k in KNOBS if mode == cb.SPGEMM_LEARNED or k != "learned_margin_pct"]
+    knobs = knobs or [k for k in KNOBS if mode == cb.SPGEMM_LEARNED or k not in LEARNED_ONLY]
 
     def run():
         best = None
@@ -99,9 +114,7 @@ def autotune(controls, a, b, blocks, mode=0, repeats=2, knobs=None, log=None, pa
             history.append((dict(trial), t))
             if log:
                 log(f"[autotune] {k}={v}: {t * 1e3:.2f} ms (best {t_cur * 1e3:.2f})")
-            if sums != ref:
-                raise AssertionError(f"tuning {trial} changed the result")
-            if t < t_cur:
+            if accept(t, t_cur, sums, ref, min_gain, str(trial)):
                 cur, t_cur = trial, t
         apply(controls, cur)
     return {"tuning": cur, "seconds": t_cur, "evaluated": len(history)}

@@ -12,6 +12,9 @@
 //    j + ncols wrap in uint32).
 //  * reduce (src/dcsr/dcsr_reduce.cpp:5-31, reduce.cl): OR over columns: one
 //    entry (column 0) per nonempty row.
+//  * digest (no counterpart; it replaces reading a matrix back to compare it
+//    on the host, as compare_matrices does, src/common/utils.cpp:42-71): a
+//    64-bit sum over the nonzeros of a mix of (position, row id, column).
 #include "clb_internal.hpp"
 
 #include "sort.hpp"
@@ -95,6 +98,34 @@ void release_dcsr(clb_dcsr &m) {
     m = clb_dcsr{};
 }
 
+// splitmix64's finaliser (Steele, Lea and Flood 2014).
+__device__ __forceinline__ u64 mix64(u64 x) {
+    u64 z = x + 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+// Sum over stored rows r and their positions q of
+// mix64(mix64(q) ^ (row << 32 | col)); one wave per row (grid-stride), one
+// atomic per workgroup.
+__global__ __launch_bounds__(256) void k_dcsr_digest(const u32 *__restrict__ rpt, const u32 *__restrict__ rows,
+                                                     const u32 *__restrict__ cols, u32 nzr,
+                                                     unsigned long long *__restrict__ out) {
+    __shared__ u64 part[4];
+    u64 acc = 0;
+    const u32 w = threadIdx.x / 64, l = threadIdx.x % 64;
+    for (u32 r = blockIdx.x * 4 + w; r < nzr; r += gridDim.x * 4) {
+        const u64 hi = (u64)rows[r] << 32;
+        const u32 e = rpt[r + 1];
+        for (u32 q = rpt[r] + l; q < e; q += 64) acc += mix64(mix64(q) ^ (hi | cols[q]));
+    }
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    if (l == 0) part[w] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(out, (unsigned long long)(part[0] + part[1] + part[2] + part[3]));
+}
+
 }  // namespace
 
 }  // namespace clb
@@ -102,6 +133,25 @@ void release_dcsr(clb_dcsr &m) {
 using namespace clb;
 
 extern "C" {
+
+clb_status clb_dcsr_digest(clb_controls ctl, const clb_dcsr *a, uint64_t *digest) {
+    return run_guarded([&] {
+        if (!a || !digest) raise(INVALID_ARGUMENT, "null argument");
+        u64 h[2] = {0, 0};
+        if (a->nzr && a->nnz) {
+            DevArray<u64> acc(ctl, 1);
+            fill_u64(ctl, acc.ptr, 1, 0ull);
+            const u32 grid = div_up(a->nzr, 4) < 8192u ? div_up(a->nzr, 4) : 8192u;
+            k_dcsr_digest<<<grid, 256, 0, ctl->stream>>>(u32p(a->rpt), u32p(a->rows), u32p(a->cols), a->nzr,
+                                                         reinterpret_cast<unsigned long long *>(acc.ptr));
+            CLB_LAUNCHED("dcsr_digest");
+            readback(ctl, h, acc.ptr, 8);
+        }
+        // the shape and sizes take part too
+        *digest = h[0] ^ (((u64)a->nnz << 32 | a->nzr) * 0x9E3779B97F4A7C15ull) ^
+                  ((u64)a->nrows << 32 | a->ncols);
+    });
+}
 
 clb_status clb_dcsr_transpose(clb_controls ctl, const clb_dcsr *a, clb_dcsr *out) {
     return run_guarded([&] {

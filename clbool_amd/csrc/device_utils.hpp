@@ -143,6 +143,53 @@ __device__ __forceinline__ uint32_t wave_incl_sum_dpp(uint32_t v) {
     return v;
 }
 
+// Segmented inclusive OR across a wave with DPP moves only: lane l ORs the
+// values of lanes [l - d, l] of its run (d = distance to the run's first
+// lane; runs are contiguous).  Shifts 1, 2, 4, 8 within rows of 16 lanes,
+// then row_bcast:15 (lane 15 / 47 into rows 1 / 3) and row_bcast:31 (lane 31
+// into rows 2 and 3), each taken only when its source lane is in the run.
+// All 64 lanes must be active.
+__device__ __forceinline__ uint32_t wave_seg_or_dpp(uint32_t v, uint32_t d, uint32_t l) {
+    uint32_t t;
+    t = dpp_row<0x111>(0u, v);
+    v |= d >= 1 ? t : 0u;
+    t = dpp_row<0x112>(0u, v);
+    v |= d >= 2 ? t : 0u;
+    t = dpp_row<0x114>(0u, v);
+    v |= d >= 4 ? t : 0u;
+    t = dpp_row<0x118>(0u, v);
+    v |= d >= 8 ? t : 0u;
+    t = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xA, 0xF, false);
+    v |= d >= (l & 15u) + 1u ? t : 0u;
+    t = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xC, 0xF, false);
+    v |= (l >= 32u && d >= l - 31u) ? t : 0u;
+    return v;
+}
+
+// ORs bit (1 << (x & 31)) into word x >> 5 of an LDS bitmap for one value
+// per lane, all 64 lanes active, values ascending across the lanes: lanes
+// whose values share a word OR their bits together first (a run of equal
+// words is contiguous), and only the run's last lane issues the LDS atomic.
+// A wave with no shared word issues one atomic per lane as usual.
+__device__ __forceinline__ void wave_or_sorted_bits(uint32_t *bm, uint32_t x) {
+    const uint32_t l = (uint32_t)lane();
+    const uint32_t w = x >> 5;
+    const uint32_t bit = 1u << (x & 31u);
+    // the word of the lane below (wave_shr:1; lane 0 gets ~0, never a word)
+    const uint32_t wprev = (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)w, 0x138, 0xF, 0xF, false);
+    const uint64_t cont = __ballot(l > 0 && wprev == w);
+    if (cont == 0) {
+        atomicOr(&bm[w], bit);
+        return;
+    }
+    const uint64_t starts = ~cont;  // bit 0 is always set
+    const uint64_t upto = starts & (l == 63u ? ~0ull : ((2ull << l) - 1ull));
+    const uint32_t s = 63u - (uint32_t)__clzll((long long)upto);
+    const uint32_t v = wave_seg_or_dpp(bit, l - s, l);
+    const bool last = l == 63u || !((cont >> (l + 1u)) & 1ull);
+    if (last) atomicOr(&bm[w], v);
+}
+
 // Number of set bits of m in the lanes below this one.
 __device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));

@@ -104,13 +104,15 @@ __global__ __launch_bounds__(256) void k_mlp_forward(const float *__restrict__ p
     y[s] = mlp_fwd(w, xs, a1, a2);
 }
 
-// Capacity of every row from the prediction.  Rows the bin rule sends to a
-// bitmap get the exact upper bound min(P, ncols) (a bitmap holds any row);
-// the rest get ceil(pred * (1 + margin)) + CAP_PAD, clamped to that bound.
+// Capacity of every row from the prediction: ceil(pred * (1 + margin)) +
+// CAP_PAD, clamped to the exact upper bound min(P, ncols); tiny rows (the
+// wave kernel holds any of them) get the bound.  Rows the bin rule sends to
+// a bitmap keep the predicted capacity when bitmap_pred is set (a bitmap row
+// over its capacity is dumped whole or re-run exactly), else the bound.
 __global__ __launch_bounds__(256) void k_predict_caps(const float *__restrict__ params,
                                                       const float *__restrict__ feat, const u32 *__restrict__ prod,
-                                                      u32 n, BinCfg c, float margin, u32 *__restrict__ cap,
-                                                      float *__restrict__ pred) {
+                                                      u32 n, BinCfg c, float margin, u32 bitmap_pred,
+                                                      u32 *__restrict__ cap, float *__restrict__ pred) {
     __shared__ float w[NPARAM];
     for (int p = threadIdx.x; p < NPARAM; p += 256) w[p] = params[p];
     __syncthreads();
@@ -132,8 +134,16 @@ __global__ __launch_bounds__(256) void k_predict_caps(const float *__restrict__ 
     float want = ceilf(est * (1.0f + margin)) + (float)CAP_PAD;
     u32 k = want >= (float)ub ? ub : (want < 1.0f ? 1u : (u32)want);
     if (P <= c.tiny_max) k = ub;  // the wave kernel holds any tiny row
-    if (classify(P, k, c) >= BIN_BITMAP0) k = ub;
+    if (!bitmap_pred && classify(P, k, c) >= BIN_BITMAP0) k = ub;
     cap[s] = k;
+}
+
+// Capacities of the one pass without a predictor: min(P, ncols) per row.
+__global__ void k_bound_caps(const u32 *__restrict__ prod, u32 n, u32 ncols, u32 *__restrict__ cap) {
+    const u32 s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const u32 P = prod[s];
+    cap[s] = P < ncols ? P : ncols;
 }
 
 // Fused forward and backward over TRAIN_WG samples per block.  Writes the
@@ -324,45 +334,81 @@ static void ensure_ready(clb_controls_s *ctl) {
               "(clb_predictor_set_params)");
 }
 
+// Capacities (and predictions) of rows [0, s.nrows) of a prepared block.
+static void learned_caps(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &b, Setup &s, bool bound_only,
+                         u32 *cap, float *pred) {
+    if (bound_only) {
+        prepare(ctl, a, b, s.row0, s.nrows, s, true);
+        k_bound_caps<<<div_up(s.nrows, 256), 256, 0, ctl->stream>>>(s.P, s.nrows, b.ncols, cap);
+        CLB_LAUNCHED("bound_caps");
+    } else {
+        ensure_ready(ctl);
+        DevArray<float> feat;
+        prepare_features(ctl, a, b, s.row0, s.nrows, s, feat);
+        const BinCfg c = learned_bincfg(ctl, b.ncols);
+        const float margin = ctl->tuning.learned_margin_pct / 100.0f;
+        const u32 bitmap_pred = ctl->tuning.reserved[CLB_TUNE_BITMAP_PRED_CAPS] ? 1u : 0u;
+        k_predict_caps<<<div_up(s.nrows, 256), 256, 0, ctl->stream>>>(state(ctl)->params.ptr, feat.ptr, s.P,
+                                                                       s.nrows, c, margin, bitmap_pred, cap, pred);
+        CLB_LAUNCHED("predict_caps");
+    }
+}
+
+void plan_sizes_learned(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &b, Setup &s, u32 *size,
+                        float *pred, bool bound_only) {
+    learned_caps(ctl, a, b, s, bound_only, size, bound_only ? nullptr : pred);
+    s.cfg = learned_bincfg(ctl, b.ncols);
+}
+
 void spgemm_learned(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &b, u32 rb, u32 re, bool timing,
-                    clb_dcsr &out, clb_spgemm_stats &st) {
-    ensure_ready(ctl);
-    Learned *L = state(ctl);
+                    clb_dcsr &out, clb_spgemm_stats &st, bool bound_only, const PlanRows *pr) {
     const u32 nrows = re - rb;
     Setup s;
-    DevArray<float> feat;
-    prepare_features(ctl, a, b, rb, nrows, s, feat);
-    s.cfg = learned_bincfg(ctl, b.ncols);
     LaunchTimer lt;
     lt.on = timing;
     auto t0 = std::chrono::steady_clock::now();
-    DevArray<u32> cap(ctl, nrows), nnz_row(ctl, nrows);
-    DevArray<float> pred(ctl, nrows);
-    const float margin = ctl->tuning.learned_margin_pct / 100.0f;
-    k_predict_caps<<<div_up(nrows, 256), 256, 0, ctl->stream>>>(L->params.ptr, feat.ptr, s.prod.ptr, nrows, s.cfg,
-                                                                 margin, cap.ptr, pred.ptr);
-    CLB_LAUNCHED("predict_caps");
+    DevArray<u32> cap_own, nnz_row(ctl, nrows);
+    DevArray<float> pred_own;
+    u32 *cap = nullptr;
+    const float *pred = nullptr;
+    if (pr) {
+        // planned block: capacities (and predictions) of all rows were
+        // computed at planning time
+        prepare(ctl, a, b, rb, nrows, s, false, pr->shared);
+        s.P = pr->prod;
+        cap = pr->size;
+        pred = pr->pred;
+    } else {
+        s.nrows = nrows;
+        s.row0 = rb;
+        cap_own.reset(ctl, nrows);
+        if (!bound_only) pred_own.reset(ctl, nrows);
+        learned_caps(ctl, a, b, s, bound_only, cap_own.ptr, pred_own.ptr);
+        cap = cap_own.ptr;
+        pred = pred_own.ptr;
+    }
+    s.cfg = learned_bincfg(ctl, b.ncols);
     DevArray<u32> perm;
     BinBuf bb;
     BinHost h;
-    bin_rows(ctl, s, cap.ptr, nnz_row.ptr, perm, bb);
+    bin_rows(ctl, s, cap, nnz_row.ptr, perm, bb);
     DevArray<u64> toff(ctl, (u64)nrows + 1);
-    exclusive_scan_u32_to_u64(ctl, cap.ptr, toff.ptr, nrows, s.scal.ptr + 4);
+    exclusive_scan_u32_to_u64(ctl, cap, toff.ptr, nrows, s.scal.ptr + 4);
     read_bins(ctl, bb, h);
     u64 sc[8];
     readback(ctl, sc, s.scal.ptr, sizeof(sc));
-    st.products = sc[0];
+    st.products = pr ? pr->products : sc[0];
     st.rows = nrows;
     st.n_bins = NBINS;
     st.capacity = sc[4];
-    if (sc[0] == 0) {
+    if (st.products == 0) {
         out.nrows = a.nrows;
         out.ncols = b.ncols;
         return;
     }
     DevArray<u32> temp(ctl, sc[4] ? sc[4] : 1);
     u32 *fails = reinterpret_cast<u32 *>(s.scal.ptr + 5);
-    Out o{nnz_row.ptr, toff.ptr, temp.ptr, cap.ptr, fails, Dump{}, ONE_PASS_PROBES};
+    Out o{nnz_row.ptr, toff.ptr, temp.ptr, cap, fails, Dump{}, ONE_PASS_PROBES};
     DumpBufs dump;
     setup_dump(ctl, s, h, dump, o, true);
     ensure_splits(ctl, s, h);
@@ -385,7 +431,7 @@ void spgemm_learned(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &b, u
     DevArray<u64> soff;
     if (nfail) {
         key2.reset(ctl, nrows);
-        k_failed_keys<<<div_up(nrows, 256), 256, 0, ctl->stream>>>(nnz_row.ptr, s.prod.ptr, nrows, b.ncols,
+        k_failed_keys<<<div_up(nrows, 256), 256, 0, ctl->stream>>>(nnz_row.ptr, s.P, nrows, b.ncols,
                                                                     key2.ptr);
         CLB_LAUNCHED("failed_keys");
         Setup &sx = s;
@@ -417,6 +463,17 @@ void spgemm_learned(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &b, u
     readback(ctl, sc2, s.scal.ptr, sizeof(sc2));
     const u64 nnz = sc2[1];
     const u32 nzr = static_cast<u32>(sc2[2]);
+    if (pr && nnz > pr->bound) {
+        // the block's result exceeds the plan's bound (rows grew past their
+        // capacities): hand the exact row sizes back for a finer cut
+        CLB_HIP(hipMemcpyAsync(pr->size, nnz_row.ptr, 4ull * nrows, hipMemcpyDeviceToDevice, ctl->stream),
+                COPY_BUFFER_ERROR);
+        sync(ctl);
+        *pr->replan = true;
+        out.nrows = a.nrows;
+        out.ncols = b.ncols;
+        return;
+    }
     if (nnz > 0xFFFFFFFFull) raise(INVALID_VALUE, "result has more nonzeros than uint32 indices allow; multiply by row blocks");
     out.nrows = a.nrows;
     out.ncols = b.ncols;
@@ -432,9 +489,9 @@ void spgemm_learned(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &b, u
         lt.begin(cs);
         k_compact_rows<<<nrows, 256, 0, cs>>>(nnz_row.ptr, offs.ptr, temp.ptr, toff.ptr, side.ptr, soff.ptr,
                                               nfail ? key2.ptr : nullptr, dump.on ? dump.isdense.ptr : nullptr,
-                                              pred.ptr, u32p(cols), err.ptr);
+                                              pred, u32p(cols), err.ptr);
         CLB_LAUNCHED("compact_rows");
-        lt.end((int)CLB_KF(CLB_KP_FALLBACK, CLB_KK_EMIT), cs);
+        lt.end((int)CLB_KF_COMPACT, cs);
         emit_dense(ctl, perm.ptr, h, dump, offs.ptr, u32p(cols), &lt, CLB_KP_ONEPASS, timing ? &st : nullptr);
         join_side(ctl);
         assemble(ctl, s, a, nnz_row.ptr, offs.ptr, pos.ptr, nnz, nzr, cols, out);
@@ -663,7 +720,7 @@ clb_status clb_predictor_predict(clb_controls ctl, const clb_dcsr *a, const clb_
         prepare_features(ctl, *a, *b, 0, a->nzr, s, feat);
         const BinCfg c = learned_bincfg(ctl, b->ncols);
         k_predict_caps<<<div_up(a->nzr, 256), 256, 0, ctl->stream>>>(state(ctl)->params.ptr, feat.ptr, s.prod.ptr,
-                                                                      a->nzr, c, 0.0f, cap.ptr, pred.ptr);
+                                                                      a->nzr, c, 0.0f, 1u, cap.ptr, pred.ptr);
         CLB_LAUNCHED("predict_caps");
         readback(ctl, pred_nnz, pred.ptr, 4ull * a->nzr);
     });

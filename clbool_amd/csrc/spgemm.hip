@@ -284,43 +284,52 @@ void ensure_splits(clb_controls_s *ctl, Setup &s, const BinHost &h) {
 }
 
 void prepare(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &b, u32 row_begin, u32 nrows, Setup &s,
-             bool count) {
+             bool count, const Setup *shared) {
     s.nrows = nrows;
     s.row0 = row_begin;
-    s.mapn = b.nrows;
-    s.brange.reset(ctl, s.mapn ? s.mapn : 1);
-    // hypersparse B (under one stored row in 16): a presence bit per row id
-    // instead of clearing the whole row map
-    const bool sparse_map = (u64)b.nzr * 16 < s.mapn;
-    if (sparse_map) {
-        s.bpres.reset(ctl, ((u64)s.mapn + 31) / 32);
-        fill_u32(ctl, s.bpres.ptr, s.bpres.n, 0u);
+    if (shared) {
+        // a plan's block: B's row map and split table were built once
+        s.mapn = shared->mapn;
+        s.g = shared->g;
+        s.cfg = shared->cfg;
+        s.b = shared->b;
     } else {
-        fill_u64(ctl, reinterpret_cast<u64 *>(s.brange.ptr), s.mapn, 0ull);
+        s.mapn = b.nrows;
+        s.brange.reset(ctl, s.mapn ? s.mapn : 1);
+        // hypersparse B (under one stored row in 16): a presence bit per row id
+        // instead of clearing the whole row map
+        const bool sparse_map = (u64)b.nzr * 16 < s.mapn;
+        if (sparse_map) {
+            s.bpres.reset(ctl, ((u64)s.mapn + 31) / 32);
+            fill_u32(ctl, s.bpres.ptr, s.bpres.n, 0u);
+        } else {
+            fill_u64(ctl, reinterpret_cast<u64 *>(s.brange.ptr), s.mapn, 0ull);
+        }
+        if (b.nzr) {
+            k_brange_set<<<div_up(b.nzr, 256), 256, 0, ctl->stream>>>(u32p(b.rpt), u32p(b.rows), b.nzr, s.mapn,
+                                                                       s.brange.ptr,
+                                                                       sparse_map ? s.bpres.ptr : nullptr);
+            CLB_LAUNCHED("brange_set");
+        }
+        s.g.bpres = sparse_map ? s.bpres.ptr : nullptr;
+        s.g.a_rpt = u32p(a.rpt);
+        s.g.a_rows = u32p(a.rows);
+        s.g.a_cols = u32p(a.cols);
+        s.g.brange = s.brange.ptr;
+        s.g.b_cols = u32p(b.cols);
+        s.g.mapn = s.mapn;
+        s.g.ncols = b.ncols;
+        s.g.emit = ctl->tuning.emit_lane_bits_min;
+        s.g.combine = ctl->tuning.reserved[CLB_TUNE_GATHER_COMBINE];
+        s.cfg = make_bincfg(ctl, b.ncols);
+        s.g.splits = nullptr;
+        s.g.nsl = 0;
+        s.g.split_bits = s.cfg.bm_bits;
+        s.b = b;  // the split table is built by ensure_splits, once bitmap rows exist
     }
-    if (b.nzr) {
-        k_brange_set<<<div_up(b.nzr, 256), 256, 0, ctl->stream>>>(u32p(b.rpt), u32p(b.rows), b.nzr, s.mapn,
-                                                                   s.brange.ptr, sparse_map ? s.bpres.ptr : nullptr);
-        CLB_LAUNCHED("brange_set");
-    }
-    s.g.bpres = sparse_map ? s.bpres.ptr : nullptr;
-    s.g.a_rpt = u32p(a.rpt);
-    s.g.a_rows = u32p(a.rows);
-    s.g.a_cols = u32p(a.cols);
-    s.g.brange = s.brange.ptr;
-    s.g.b_cols = u32p(b.cols);
-    s.g.mapn = s.mapn;
     s.g.row0 = row_begin;
-    s.g.ncols = b.ncols;
-    s.g.emit = ctl->tuning.emit_lane_bits_min;
-    s.cfg = make_bincfg(ctl, b.ncols);
-    const u64 span = 32ull << s.cfg.bm_bits;
-    const u64 nsl = ((u64)b.ncols + span - 1) / span;
-    s.g.splits = nullptr;
-    s.g.nsl = 0;
-    s.g.split_bits = s.cfg.bm_bits;
-    s.b = b;  // the split table is built by ensure_splits, once bitmap rows exist
     s.prod.reset(ctl, nrows);
+    s.P = s.prod.ptr;
     s.ctr.reset(ctl, 4);
     s.scal.reset(ctl, 8);  // [0] total products, [1] nnz total, [2] nzr total, [3] overflow flag
     fill_u64(ctl, s.scal.ptr, 8, 0ull);
@@ -339,12 +348,12 @@ void bin_rows(clb_controls_s *ctl, const Setup &s, const u32 *key, u32 *nnz_row_
     fill_u32(ctl, bb.buf.ptr, bb.buf.n, 0u);
     perm.reset(ctl, s.nrows ? s.nrows : 1);
     const u32 grid = div_up(s.nrows, 256);
-    k_bin_hist<<<grid, 256, 0, ctl->stream>>>(s.g, s.prod.ptr, key, s.nrows, s.cfg, bb.hist(), bb.sums(),
+    k_bin_hist<<<grid, 256, 0, ctl->stream>>>(s.g, s.P, key, s.nrows, s.cfg, bb.hist(), bb.sums(),
                                               nnz_row_zero);
     CLB_LAUNCHED("bin_hist");
     k_bin_offsets<<<1, 64, 0, ctl->stream>>>(bb.hist(), bb.cursor());
     CLB_LAUNCHED("bin_offsets");
-    k_bin_scatter<<<grid, 256, 0, ctl->stream>>>(s.prod.ptr, key, s.nrows, s.cfg, bb.cursor(), perm.ptr);
+    k_bin_scatter<<<grid, 256, 0, ctl->stream>>>(s.P, key, s.nrows, s.cfg, bb.cursor(), perm.ptr);
     CLB_LAUNCHED("bin_scatter");
 }
 
@@ -591,42 +600,59 @@ void scan_sizes(clb_controls_s *ctl, Setup &s, const u32 *nnz_row, DevArray<u64>
 }
 
 void spgemm_plain(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &b, u32 row_begin, u32 row_end,
-                  bool timing, clb_dcsr &out, clb_spgemm_stats &st) {
+                  bool timing, clb_dcsr &out, clb_spgemm_stats &st, const PlanRows *pr) {
     Timer tm(timing, ctl->stream);
     tm.mark();  // 0
     const u32 nrows = row_end - row_begin;
     Setup s;
-    prepare(ctl, a, b, row_begin, nrows, s);
+    prepare(ctl, a, b, row_begin, nrows, s, pr == nullptr, pr ? pr->shared : nullptr);
     LaunchTimer lt;
     lt.on = timing;
-    DevArray<u32> perm;
-    BinBuf bb;
-    BinHost h1;
-    DevArray<u32> nnz_row(ctl, nrows);
-    bin_rows(ctl, s, nullptr, nnz_row.ptr, perm, bb);
-    read_bins(ctl, bb, h1);
-    u64 total_products = 0;
-    readback(ctl, &total_products, s.scal.ptr, 8);
-    st.products = total_products;
     st.rows = nrows;
     st.n_bins = NBINS;
-    tm.mark();  // 1
-    if (total_products == 0) {
+    DevArray<u32> nnz_row_own;
+    u32 *nnz_row = nullptr;
+    if (pr) {
+        // planned block: products and exact row sizes come from the plan
+        s.P = pr->prod;
+        nnz_row = pr->size;
+        st.products = pr->products;
+        tm.mark();  // 1
+        tm.mark();  // 2
+    } else {
+        DevArray<u32> perm;
+        BinBuf bb;
+        BinHost h1;
+        nnz_row_own.reset(ctl, nrows);
+        nnz_row = nnz_row_own.ptr;
+        bin_rows(ctl, s, nullptr, nnz_row, perm, bb);
+        read_bins(ctl, bb, h1);
+        u64 total_products = 0;
+        readback(ctl, &total_products, s.scal.ptr, 8);
+        st.products = total_products;
+        tm.mark();  // 1
+        if (total_products == 0) {
+            out.nrows = a.nrows;
+            out.ncols = b.ncols;
+            return;
+        }
+        Out o1{nnz_row, nullptr, nullptr, nullptr, reinterpret_cast<u32 *>(s.scal.ptr + 3), Dump{}, 0};
+        ensure_splits(ctl, s, h1);
+        launch_bins(ctl, s, perm.ptr, h1.off, SYM, o1, &lt, -1);
+        tm.mark();  // 2
+    }
+    if (st.products == 0) {
         out.nrows = a.nrows;
         out.ncols = b.ncols;
         return;
     }
-    Out o1{nnz_row.ptr, nullptr, nullptr, nullptr, reinterpret_cast<u32 *>(s.scal.ptr + 3), Dump{}, 0};
-    ensure_splits(ctl, s, h1);
-    launch_bins(ctl, s, perm.ptr, h1.off, SYM, o1, &lt, -1);
-    tm.mark();  // 2
     DevArray<u64> offs;
     DevArray<u32> pos;
-    scan_sizes(ctl, s, nnz_row.ptr, offs, pos);
+    scan_sizes(ctl, s, nnz_row, offs, pos);
     DevArray<u32> perm2;
     BinBuf bb2;
     BinHost h2;
-    bin_rows(ctl, s, nnz_row.ptr, nullptr, perm2, bb2);
+    bin_rows(ctl, s, nnz_row, nullptr, perm2, bb2);
     read_bins(ctl, bb2, h2);
     for (u32 k = 0; k < NBINS; ++k) {
         st.bin_rows[k] = h2.off[k + 1] - h2.off[k];
@@ -650,14 +676,14 @@ void spgemm_plain(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &b, u32
     if (nnz == 0) return;
     clb_buffer_s *cols = buffer_new(ctl, 4ull * nnz);
     try {
-        Out o2{nnz_row.ptr, offs.ptr, u32p(cols), nullptr, reinterpret_cast<u32 *>(s.scal.ptr + 3), Dump{}, 0};
+        Out o2{nnz_row, offs.ptr, u32p(cols), nullptr, reinterpret_cast<u32 *>(s.scal.ptr + 3), Dump{}, 0};
         DumpBufs dump;
         setup_dump(ctl, s, h2, dump, o2, false);
         ensure_splits(ctl, s, h2);
         launch_bins(ctl, s, perm2.ptr, h2.off, NUM, o2, &lt, -1);
         emit_dense(ctl, perm2.ptr, h2, dump, offs.ptr, u32p(cols), &lt, CLB_KP_NUMERIC, timing ? &st : nullptr);
         tm.mark();  // 4
-        assemble(ctl, s, a, nnz_row.ptr, offs.ptr, pos.ptr, nnz, nzr, cols, out);
+        assemble(ctl, s, a, nnz_row, offs.ptr, pos.ptr, nnz, nzr, cols, out);
         tm.mark();  // 5
         u64 flag = 0;
         readback(ctl, &flag, s.scal.ptr + 3, 8);
@@ -678,6 +704,21 @@ void spgemm_plain(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &b, u32
         st.ms_assemble = tm.ms(2, 3) + tm.ms(4, 5);
         st.ms_total = tm.ms(0, 5);
     }
+}
+
+void plan_sizes_plain(clb_controls_s *ctl, Setup &s, u32 *size) {
+    // the symbolic pass of spgemm_plain over every stored row of A
+    DevArray<u32> perm;
+    BinBuf bb;
+    BinHost h;
+    bin_rows(ctl, s, nullptr, size, perm, bb);
+    read_bins(ctl, bb, h);
+    Out o{size, nullptr, nullptr, nullptr, reinterpret_cast<u32 *>(s.scal.ptr + 3), Dump{}, 0};
+    ensure_splits(ctl, s, h);
+    launch_bins(ctl, s, perm.ptr, h.off, SYM, o, nullptr, -1);
+    u64 flag = 0;
+    readback(ctl, &flag, s.scal.ptr + 3, 8);
+    if (flag) raise(RUN_KERNEL_FAILURE, "hash table overflow in the symbolic pass");
 }
 
 void check_dims(const clb_dcsr &a, const clb_dcsr &b) {
@@ -705,8 +746,8 @@ void spgemm_dispatch(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &b, 
         return;
     }
     auto t0 = std::chrono::steady_clock::now();
-    if (o.mode == CLB_SPGEMM_LEARNED)
-        spgemm_learned(ctl, a, b, rb, re, o.timing != 0, r, st);
+    if (o.mode == CLB_SPGEMM_LEARNED || o.mode == CLB_SPGEMM_BOUND)
+        spgemm_learned(ctl, a, b, rb, re, o.timing != 0, r, st, o.mode == CLB_SPGEMM_BOUND);
     else if (o.mode == CLB_SPGEMM_PLAIN)
         spgemm_plain(ctl, a, b, rb, re, o.timing != 0, r, st);
     else

@@ -20,6 +20,22 @@ struct Setup {
     Mats g{};
     BinCfg cfg{};
     clb_dcsr b{};  // the B operand (its buffers stay owned by the caller)
+    u32 *P = nullptr;  // products per row of the block: prod.ptr, or a plan's array
+};
+
+// What a plan (plan.hip) hands to one row block's run: the block's slices of
+// the per-row arrays computed over all of A at planning time, and the B row
+// map and split table built once for all blocks.
+struct PlanRows {
+    const Setup *shared;  // B row map, presence bits, split table, bin config
+    u32 *prod;            // products of the block's rows
+    u32 *size;            // plain: exact row sizes; learned / bound: capacities
+    const float *pred;    // learned: predicted sizes (nullptr otherwise)
+    u64 products;         // sum of prod over the block
+    u64 bound;            // one pass: a result above this many nonzeros is not
+                          // assembled; the exact row sizes are written to
+                          // `size` and *replan is set instead
+    bool *replan;
 };
 
 // HIP-event timing of individual kernel launches, grouped by family
@@ -67,8 +83,10 @@ void emit_dense(clb_controls_s *ctl, const u32 *perm, const BinHost &h, DumpBufs
 BinCfg make_bincfg(const clb_controls_s *ctl, u32 ncols);
 void ensure_splits(clb_controls_s *ctl, Setup &s, const BinHost &h);
 // B row map, kernel arguments and (count == true) per-row product counts.
+// With `shared`, the B row map, split table and bin config are taken from
+// it (a plan's Setup) instead of being built.
 void prepare(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &b, u32 row_begin, u32 nrows, Setup &s,
-             bool count = true);
+             bool count = true, const Setup *shared = nullptr);
 // Bins the rows of the block by (P, key) into perm (device only, no sync).
 void bin_rows(clb_controls_s *ctl, const Setup &s, const u32 *key, u32 *nnz_row_zero, DevArray<u32> &perm,
               BinBuf &bb);
@@ -87,11 +105,22 @@ void assemble(clb_controls_s *ctl, const Setup &s, const clb_dcsr &a, const u32 
               const u32 *pos, u64 nnz, u32 nzr, clb_buffer_s *cols, clb_dcsr &out);
 void check_dims(const clb_dcsr &a, const clb_dcsr &b);
 
+// pr != nullptr: the block of a plan (the count and symbolic pass were run
+// over all rows at planning time; pr->size holds the exact row sizes).
 void spgemm_plain(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &b, u32 row_begin, u32 row_end,
-                  bool timing, clb_dcsr &out, clb_spgemm_stats &st);
-// learned.hip
+                  bool timing, clb_dcsr &out, clb_spgemm_stats &st, const PlanRows *pr = nullptr);
+// Planning passes over all stored rows of A (s from prepare over all rows):
+// exact row sizes by the symbolic pass (plain) into size.
+void plan_sizes_plain(clb_controls_s *ctl, Setup &s, u32 *size);
+// learned.hip.  bound_only: capacities min(P, ncols) for every row, no
+// predictor (CLB_SPGEMM_BOUND).
 void spgemm_learned(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &b, u32 row_begin, u32 row_end,
-                    bool timing, clb_dcsr &out, clb_spgemm_stats &st);
+                    bool timing, clb_dcsr &out, clb_spgemm_stats &st, bool bound_only = false,
+                    const PlanRows *pr = nullptr);
+// Planning pass of the one pass: s prepared over all rows (products in s.P
+// afterwards), capacities into size, predictions into pred (learned only).
+void plan_sizes_learned(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &b, Setup &s, u32 *size,
+                        float *pred, bool bound_only);
 void spgemm_dispatch(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &b, const clb_spgemm_opts &o,
                      clb_dcsr &out, clb_spgemm_stats &st);
 

@@ -46,6 +46,7 @@ struct Mats {
     u32 row0;             // first A stored row of the block
     u32 ncols;            // columns of B (and C)
     u32 emit;             // bitmap emission: lane-per-bit threshold (EMIT_DEFAULT when 0)
+    u32 combine;          // bitmap gather: OR bits of one word within a wave before the LDS atomic
     // Column slices of the bitmap kernels (nsl > 1 when ncols exceeds one
     // window of 2^split_bits words): splits[k * (nsl + 1) + s] is the
     // position in b_cols of B row k's first column in slice s (s = nsl: the
@@ -106,7 +107,9 @@ __device__ __forceinline__ u32 classify(u32 P, u32 key, const BinCfg &c) {
 // Product enumeration of one A row [js, je) by a workgroup of WG threads.
 // `big` is LDS scratch of BIG_WORDS(WG) words, 8-byte aligned (unused when
 // WG == 64).  f(col) is called once per product, possibly from divergent
-// lanes.
+// lanes; ff(col) instead of f for the products of full workgroup-path
+// windows, where it is called by all 64 lanes of a wave at once with the
+// columns ascending across the lanes.
 //
 // The A row is taken in chunks of WG nonzeros; each B row of the chunk goes
 // one of three ways by its length:
@@ -126,8 +129,8 @@ __device__ __forceinline__ u32 classify(u32 P, u32 key, const BinCfg &c) {
 constexpr int UNROLL = 4;
 __host__ __device__ constexpr int BIG_WORDS(int WG) { return 3 * WG + 2 * (WG / 64 + 2); }
 
-template <int WG, int UB, u32 BIGMIN, typename F>
-__device__ __forceinline__ void for_each_product(const Mats &g, u32 js, u32 je, u32 *big, F &&f,
+template <int WG, int UB, u32 BIGMIN, typename F, typename FF>
+__device__ __forceinline__ void for_each_product(const Mats &g, u32 js, u32 je, u32 *big, F &&f, FF &&ff,
                                                  u32 slice = NOSLICE, u32 sstep = 1) {
     const int l = dev::lane();
     constexpr int NW = WG / dev::WAVE;
@@ -265,8 +268,9 @@ __device__ __forceinline__ void for_each_product(const Mats &g, u32 js, u32 je, 
             };
             auto apply = [&](const u32(&c)[UB], u32 n) {
                 if (n == WIN) {
+                    // all 64 lanes hold consecutive columns of one B row
 #pragma unroll
-                    for (int u = 0; u < UB; ++u) f(c[u]);
+                    for (int u = 0; u < UB; ++u) ff(c[u]);
                 } else {
 #pragma unroll
                     for (int u = 0; u < UB; ++u)
@@ -285,6 +289,13 @@ __device__ __forceinline__ void for_each_product(const Mats &g, u32 js, u32 je, 
             __syncthreads();  // before the next chunk rewrites the list
         }
     }
+}
+
+// The same with one callable for every product.
+template <int WG, int UB, u32 BIGMIN, typename F>
+__device__ __forceinline__ void for_each_product(const Mats &g, u32 js, u32 je, u32 *big, F &&f,
+                                                 u32 slice = NOSLICE, u32 sstep = 1) {
+    for_each_product<WG, UB, BIGMIN>(g, js, je, big, f, f, slice, sstep);
 }
 
 // ---------------------------------------------------------------------------
@@ -631,17 +642,23 @@ __device__ __forceinline__ void bitmap_row(const Mats &g, const u32 *__restrict_
             for (u32 t = threadIdx.x; t < W / 4; t += WG) bm4[t] = make_uint4(0, 0, 0, 0);
         }
         __syncthreads();
-        if (nslices == 1)
-            for_each_product<WG, (MODE == SYM ? 16 : 8), LONG_ROW>(
-                g, js, je, big, [&](u32 col) { atomicOr(&bm[col >> 5], 1u << (col & 31)); });
-        else  // only the slice's part of each B row is read
-            for_each_product<WG, (MODE == SYM ? 16 : 8), LONG_ROW>(
-                g, js, je, big,
-                [&](u32 col) {
-                    const u64 d = (u64)col - base;
-                    if (d < SPAN) atomicOr(&bm[(u32)(d >> 5)], 1u << (col & 31));
-                },
-                use_splits ? s : NOSLICE, sstep);
+        auto set1 = [&](u32 col) { atomicOr(&bm[col >> 5], 1u << (col & 31)); };
+        auto setw = [&](u32 col) { dev::wave_or_sorted_bits(bm, col); };
+        auto setd = [&](u32 col) {
+            const u64 d = (u64)col - base;
+            if (d < SPAN) atomicOr(&bm[(u32)(d >> 5)], 1u << (col & 31));
+        };
+        // a slice read through the split table holds only its own columns
+        auto setdw = [&](u32 col) { dev::wave_or_sorted_bits(bm, (u32)((u64)col - base)); };
+        constexpr int UBW = MODE == SYM ? 16 : 8;
+        if (nslices == 1) {
+            if (g.combine) for_each_product<WG, UBW, LONG_ROW>(g, js, je, big, set1, setw);
+            else for_each_product<WG, UBW, LONG_ROW>(g, js, je, big, set1);
+        } else if (use_splits && g.combine) {  // only the slice's part of each B row is read
+            for_each_product<WG, UBW, LONG_ROW>(g, js, je, big, setd, setdw, s, sstep);
+        } else {
+            for_each_product<WG, UBW, LONG_ROW>(g, js, je, big, setd, use_splits ? s : NOSLICE, sstep);
+        }
         __syncthreads();
         if (MODE != SYM && WBITS >= 14 && o.dump.store) {
             // exact count per 1024-word chunk.  Slice 0 decides for the whole
@@ -660,7 +677,9 @@ __device__ __forceinline__ void bitmap_row(const Mats &g, const u32 *__restrict_
 #pragma unroll
             for (u32 off2 = TPC / 2; off2 > 0; off2 >>= 1) csum += __shfl_xor(csum, off2, TPC);
             const u32 n = dev::block_sum<u32, WG>(cnt, red);
-            if (s == 0) dumping = n >= o.dump.dense_min;
+            // a one-pass row over its capacity already in its first window
+            // is dumped too: a dumped row needs no temporary slot
+            if (s == 0) dumping = n >= o.dump.dense_min || (MODE == ONE && n > lim);
             if (dumping) {
                 const u32 slot = idx;
                 uint4 *dst = o.dump.store + ((u64)slot * nslices + s) * (W / 4);
@@ -720,10 +739,11 @@ __device__ __forceinline__ void bitmap_row(const Mats &g, const u32 *__restrict_
         const u32 n = dev::block_sum<u32, WG>(mine, red);
         if (threadIdx.x == 0) o.nnz_row[r] = n;
     } else if (dumping) {
-        // listed for emission only when it fits its slot: the numeric pass
-        // needs the symbolic count, the one pass its capacity
+        // listed for emission: in the one pass always (it is written at its
+        // exact offset, its capacity does not matter); in the numeric pass
+        // when it matches the symbolic count
         if (threadIdx.x == 0) {
-            const bool fits = MODE == ONE ? written <= lim : written == lim;
+            const bool fits = MODE == ONE ? written <= 0xFFFFFFFFull : written == lim;
             if (fits) {
                 o.dump.list[atomicAdd(o.dump.count, 1u)] = idx;
                 o.dump.isdense[r] = 1;

@@ -50,24 +50,51 @@ def uniform_pairs(n, nrows, ncols, seed):
     return (have >> np.uint64(32)).astype(np.uint32), (have & np.uint64(0xFFFFFFFF)).astype(np.uint32)
 
 
-def rmat(scale, edge_factor, seed, a=0.57, b=0.19, c=0.19, chunk=1 << 22, n_edges=None):
-    """R-MAT graph on 2^scale vertices: sorted unique pairs without self-loops."""
+def _rmat_bits(x, a, ab, abc):
+    """Row and column ids from the per-level uniform draws x[level, edge]."""
+    k = x.shape[1]
+    r = np.zeros(k, np.uint32)
+    q = np.zeros(k, np.uint32)
+    for lv in range(x.shape[0]):
+        xl = x[lv]
+        rb = xl >= ab
+        cb = np.where(rb, xl >= abc, xl >= a)
+        r = (r << np.uint32(1)) | rb.astype(np.uint32)
+        q = (q << np.uint32(1)) | cb.astype(np.uint32)
+    keep = r != q
+    return (r[keep].astype(np.uint64) << np.uint64(32)) | q[keep].astype(np.uint64)
+
+
+def _rmat_chunk(state, skip, scale, k, a, ab, abc):
+    """The chunk whose draws start `skip` 64-bit outputs into the stream."""
+    bg = np.random.PCG64()
+    bg.state = state
+    bg.advance(skip)
+    return _rmat_bits(np.random.Generator(bg).random((scale, k), dtype=np.float32), a, ab, abc)
+
+
+def rmat(scale, edge_factor, seed, a=0.57, b=0.19, c=0.19, chunk=1 << 22, n_edges=None, threads=8):
+    """R-MAT graph on 2^scale vertices: sorted unique pairs without self-loops.
+    Edge e of the chunk starting at edge s0 (chunk length k) takes level l's
+    quadrant from float32 draw number s0 * scale + l * k + (e - s0) of the
+    seeded PCG64 stream.  A float32 draw takes half of one 64-bit output, and
+    every chunk before the last holds an even number of draws, so each chunk
+    starts its own copy of the stream advanced to its first draw and the
+    chunks are drawn in parallel with the same result as drawing in order."""
+    import concurrent.futures as cf
     rng = np.random.default_rng(seed)
     m = int(n_edges if n_edges is not None else edge_factor * (1 << scale))
     ab, abc = a + b, a + b + c
-    keys = []
-    for s0 in range(0, m, chunk):
-        k = min(chunk, m - s0)
-        r = np.zeros(k, np.uint32)
-        q = np.zeros(k, np.uint32)
-        for _ in range(scale):
-            x = rng.random(k, dtype=np.float32)
-            rb = x >= ab
-            cb = np.where(rb, x >= abc, x >= a)
-            r = (r << np.uint32(1)) | rb.astype(np.uint32)
-            q = (q << np.uint32(1)) | cb.astype(np.uint32)
-        keep = r != q
-        keys.append((r[keep].astype(np.uint64) << np.uint64(32)) | q[keep].astype(np.uint64))
+    starts = list(range(0, m, chunk))
+    state = rng.bit_generator.state
+    if (scale * chunk) % 2:
+        raise ValueError("chunks must hold an even number of draws")
+    jobs = [(state, s0 * scale // 2, scale, min(chunk, m - s0), a, ab, abc) for s0 in starts]
+    if len(jobs) <= 1:
+        keys = [_rmat_chunk(*j) for j in jobs]
+    else:
+        with cf.ThreadPoolExecutor(max_workers=max(1, threads)) as ex:
+            keys = list(ex.map(lambda j: _rmat_chunk(*j), jobs))
     key = np.unique(np.concatenate(keys)) if keys else np.zeros(0, np.uint64)
     return (key >> np.uint64(32)).astype(np.uint32), (key & np.uint64(0xFFFFFFFF)).astype(np.uint32)
 

@@ -81,15 +81,29 @@ typedef struct clb_tuning {
     uint32_t bitmap_persistent;    /* nonzero: the bitmap rows run on a persistent
                                       grid that takes rows from a counter
                                       instead of one workgroup per row          */
-    uint32_t reserved[8];
+    uint32_t reserved[8];          /* reserved[CLB_TUNE_BITMAP_PRED_CAPS]: nonzero:
+                                      the one pass sizes bitmap rows by their
+                                      predicted capacity too (a bitmap row over
+                                      it is dumped whole, or re-run exactly),
+                                      0: by min(products, ncols)
+                                      reserved[CLB_TUNE_GATHER_COMBINE]: nonzero:
+                                      the bitmap gather ORs the bits that lanes
+                                      of a wave set in one 32-bit word together
+                                      and issues one LDS atomic per word        */
 } clb_tuning;
+#define CLB_TUNE_BITMAP_PRED_CAPS 0
+#define CLB_TUNE_GATHER_COMBINE   1
 
 #define CLB_SPGEMM_PLAIN   0u  /* two-pass: symbolic count, scan, numeric      */
 #define CLB_SPGEMM_LEARNED 1u  /* one pass sized by the learned predictor,
                                   overflowing rows re-run exactly              */
+#define CLB_SPGEMM_BOUND   2u  /* one pass with every row sized by its upper
+                                  bound min(products, ncols): no predictor,
+                                  never a re-run (the learned mode without
+                                  its network)                                 */
 
 typedef struct clb_spgemm_opts {
-    uint32_t mode;       /* CLB_SPGEMM_PLAIN or CLB_SPGEMM_LEARNED             */
+    uint32_t mode;       /* CLB_SPGEMM_PLAIN, _LEARNED or _BOUND               */
     uint32_t row_begin;  /* range of A's stored rows to multiply (row block);  */
     uint32_t row_end;    /* row_end == 0 means all rows                       */
     uint32_t timing;     /* nonzero: record per-stage times (HIP events)      */
@@ -121,13 +135,17 @@ typedef struct clb_spgemm_stats {
 /* kernel families of clb_spgemm_stats.ms_kernel: pass x kind */
 #define CLB_KP_SYMBOLIC 0u  /* plain mode, symbolic pass               */
 #define CLB_KP_NUMERIC  1u  /* plain mode, numeric pass                */
-#define CLB_KP_ONEPASS  2u  /* learned mode, the single sized pass     */
+#define CLB_KP_ONEPASS  2u  /* learned / bound mode, the single sized pass */
 #define CLB_KP_FALLBACK 3u  /* learned mode, exact re-run of overflows */
 #define CLB_KK_TINY     0u
 #define CLB_KK_HASH     1u
 #define CLB_KK_BITMAP   2u
-#define CLB_KK_EMIT     3u  /* bitmap emission / row compaction        */
+#define CLB_KK_EMIT     3u  /* emission of dumped dense bitmap rows    */
 #define CLB_KF(pass, kind) ((pass) * 4u + (kind))
+/* The re-run never dumps rows (it has no emission launch), so its EMIT slot
+ * holds the one pass's compaction instead: every row copied from its
+ * temporary slot to its exact position. */
+#define CLB_KF_COMPACT CLB_KF(CLB_KP_FALLBACK, CLB_KK_EMIT)
 
 typedef struct clb_device_info {
     char     name[256];
@@ -218,6 +236,38 @@ clb_status clb_dcsr_spgemm_hash(clb_controls ctl, const clb_dcsr *a, const clb_d
  * modes. */
 clb_status clb_dcsr_spgemm(clb_controls ctl, const clb_dcsr *a, const clb_dcsr *b, const clb_spgemm_opts *opts,
                            clb_dcsr *out, clb_spgemm_stats *stats);
+
+/* ---- a product taken in row blocks (no counterpart in the reference: its
+ *      matrix_multiplication_hash wraps around past 2^32 - 1 nonzeros,
+ *      hash.cpp:312,318).  The plan runs each mode's sizing pass over all of
+ *      A's stored rows and cuts them into consecutive blocks whose results
+ *      hold at most block_bound nonzeros (0 or >= 2^32 - 1: 2^32 - 1):
+ *      PLAIN by exact sizes from the symbolic pass (a block then runs the
+ *      numeric pass only), LEARNED by predicted capacities, BOUND by
+ *      min(products, ncols).  A LEARNED block whose exact result exceeds the
+ *      bound is cut again by the exact sizes it found and run again, so
+ *      nblocks can grow during the runs: read it again after each run.  The
+ *      plan keeps references to A and B.  Block k is stored rows
+ *      [row_begin, row_end) of A, and its result is the same as
+ *      clb_dcsr_spgemm over that row block. */
+typedef struct clb_spgemm_plan_s *clb_spgemm_plan;
+typedef struct clb_spgemm_plan_info {
+    uint32_t mode;
+    uint32_t nblocks;      /* current number of blocks                         */
+    uint32_t replans;      /* blocks cut again after exceeding the bound       */
+    uint32_t reserved;
+    uint64_t products;     /* intermediate products of the whole product       */
+    uint64_t planned_nnz;  /* sum of the sizes the cut used                    */
+    double   ms_plan;      /* host time of clb_spgemm_plan_create              */
+} clb_spgemm_plan_info;
+clb_status clb_spgemm_plan_create(clb_controls ctl, const clb_dcsr *a, const clb_dcsr *b, uint32_t mode,
+                                  uint64_t block_bound, clb_spgemm_plan *out);
+clb_status clb_spgemm_plan_info_get(clb_spgemm_plan plan, clb_spgemm_plan_info *info);
+clb_status clb_spgemm_plan_block(clb_spgemm_plan plan, uint32_t k, uint32_t *row_begin, uint32_t *row_end);
+clb_status clb_spgemm_plan_run(clb_controls ctl, clb_spgemm_plan plan, uint32_t k, uint32_t timing, clb_dcsr *out,
+                               clb_spgemm_stats *stats);
+clb_status clb_spgemm_plan_release(clb_spgemm_plan plan);
+
 /* dcsr::kronecker_product (src/dcsr/dcsr_kronecker_product.cpp:6-65) */
 clb_status clb_dcsr_kronecker(clb_controls ctl, const clb_dcsr *a, const clb_dcsr *b, clb_dcsr *out);
 /* dcsr::transpose (src/dcsr/dcsr_transpose.cpp:5-17): an empty input gives
@@ -232,6 +282,13 @@ clb_status clb_dcsr_submatrix(clb_controls ctl, const clb_dcsr *a, uint32_t i, u
  * nonempty row; shape a.nrows x 1 (the reference labels it a.nzr x 1).  An
  * empty input gives a 0 x 0 matrix. */
 clb_status clb_dcsr_reduce(clb_controls ctl, const clb_dcsr *a, clb_dcsr *out);
+/* A 64-bit digest of a DCSR matrix computed on the device: the sum over its
+ * nonzeros of a mix of (position, row id, column), combined with its shape
+ * and sizes.  Equal matrices have equal digests; a changed, moved, missing or
+ * extra entry changes it except with probability about 2^-64.  No
+ * counterpart in the reference: it replaces reading matrices back to compare
+ * them on the host (compare_matrices, src/common/utils.cpp:42-71). */
+clb_status clb_dcsr_digest(clb_controls ctl, const clb_dcsr *a, uint64_t *digest);
 /* coo::matrix_addition (src/coo/coo_matrix_addition.cpp:32-73) */
 clb_status clb_coo_addition(clb_controls ctl, const clb_coo *a, const clb_coo *b, clb_coo *out);
 /* csr::matrix_addition (src/csr/csr_addition.cpp:26-168) */

@@ -1,13 +1,16 @@
 """The benchmark workload at its full size: R-MAT scale 20, edge factor 16,
 seed 1, A * A in row blocks of at most 2^31 products (its 9.7e9 output
-nonzeros exceed one uint32 DCSR).  The oracle cannot redo the whole product
-in a test, so the full-size checks are size-independent properties:
+nonzeros exceed one uint32 DCSR).  Every row of the product is checked:
 
-* plain two-pass and learned one-pass results agree: row pointers and row ids
-  in full, columns in three 2^20-wide windows of every block;
-* every row's size is at most min(products of the row, ncols), rows are
-  strictly ascending, and the blocks' products add up to the workload total;
-* 48 seeded rows per block equal the oracle's Gustavson product exactly.
+* the oracle's per-row digests (column count and an order-independent 64-bit
+  hash of the columns, oracle.spgemm_digest: Gustavson without materialising
+  the product) of all 546,835 stored rows of A equal the digests of the GPU
+  result read back in chunks, for the plain two-pass and the learned one-pass
+  modes; the GPU rows are exactly A's rows with a nonempty product and every
+  row's columns are strictly ascending;
+* row pointers and row ids of the two modes agree in full, the blocks'
+  products add up to the workload total, and the product does not fit one
+  uint32 DCSR.
 """
 import numpy as np
 import pytest
@@ -18,6 +21,8 @@ import oracle
 from clbool_amd import generators as gen
 
 pytestmark = pytest.mark.gpu
+
+CHUNK = 1 << 28  # columns read back per chunk (1 GiB)
 
 
 @pytest.fixture(scope="module")
@@ -31,43 +36,53 @@ def rmat20(controls):
     return A, a, prod, bench.row_blocks(prod, 1 << 31)
 
 
-def _window_cols(ctl, m, k=1 << 20):
-    k = min(m.nnz, k)
-    return [m.cols_gpu.read(ctl, k, offset=4 * off) for off in (0, (m.nnz - k) // 2, m.nnz - k)]
+def _gpu_digests(ctl, m):
+    """(rpt, rows, per-row digests, rows with unsorted columns) of a DCSR
+    block on the device, its columns read back CHUNK at a time."""
+    rpt = m.rpt_gpu.read(ctl, m.nzr + 1).astype(np.int64)
+    rows = m.rows_gpu.read(ctl, m.nzr)
+    h = np.zeros(m.nzr, np.uint64)
+    bad = 0
+    r = 0
+    while r < m.nzr:
+        start = int(rpt[r])
+        r2 = int(np.searchsorted(rpt, start + CHUNK, side="right")) - 1
+        r2 = min(max(r2, r + 1), m.nzr)
+        cols = m.cols_gpu.read(ctl, int(rpt[r2]) - start, offset=4 * start)
+        hh, b = oracle.dcsr_row_digest(rpt.astype(np.uint32), cols, base=start, r0=r, r1=r2)
+        h[r:r2] = hh
+        bad += b
+        r = r2
+    return rpt, rows, h, bad
 
 
-def test_rmat20_full_product_properties(controls, rmat20):
+def test_rmat20_every_row_matches_oracle(controls, rmat20):
     A, a, prod, blocks = rmat20
     bench.train_predictor(cb, gen, controls, lambda *x: None)
-    rng = np.random.default_rng(7)
-    total_products, total_nnz = 0, 0
+    total_products, total_nnz, checked_rows = 0, 0, 0
     for lo, hi in blocks:
-        P, st = cb.dcsr.spgemm(controls, a, a, mode=cb.SPGEMM_PLAIN, row_begin=lo, row_end=hi)
-        L, sl = cb.dcsr.spgemm(controls, a, a, mode=cb.SPGEMM_LEARNED, row_begin=lo, row_end=hi)
-        assert st["products"] == sl["products"] == int(prod[lo:hi].sum())
-        total_products += st["products"]
-        assert (P.nnz, P.nzr) == (L.nnz, L.nzr), f"block {lo}: plain and learned sizes differ"
-        prpt, prows = P.rpt_gpu.read(controls, P.nzr + 1), P.rows_gpu.read(controls, P.nzr)
-        assert np.array_equal(prpt, L.rpt_gpu.read(controls, L.nzr + 1))
-        assert np.array_equal(prows, L.rows_gpu.read(controls, L.nzr))
-        for x, y in zip(_window_cols(controls, P), _window_cols(controls, L)):
-            assert np.array_equal(x, y), f"block {lo}: column windows differ"
-        # structure: ascending rows, each of them a row of the block, sizes
-        # bounded by the row's products and the width
-        assert np.all(np.diff(prows.astype(np.int64)) > 0)
-        pos = np.searchsorted(A.rows, prows)
-        assert np.all((pos >= lo) & (pos < hi)) and np.array_equal(A.rows[pos], prows)
-        lens = np.diff(prpt.astype(np.int64))
-        assert np.all(lens >= 1) and np.all(lens <= np.minimum(prod[pos], 1 << 20))
-        # exact rows against the oracle
-        sel = np.sort(rng.choice(np.arange(lo, hi), size=min(48, hi - lo), replace=False)).astype(np.uint32)
-        ref = oracle.spgemm(A, A, row_sel=sel)
-        for q in range(ref.nzr):
-            i = int(np.searchsorted(prows, ref.rows[q]))
-            assert i < P.nzr and prows[i] == ref.rows[q]
-            got = P.cols_gpu.read(controls, int(prpt[i + 1] - prpt[i]), offset=4 * int(prpt[i]))
-            assert np.array_equal(got, ref.cols[ref.rpt[q]:ref.rpt[q + 1]]), f"row {ref.rows[q]}"
-        total_nnz += P.nnz
-        del P, L
+        cnt, href = oracle.spgemm_digest(A, A, lo, hi)
+        nz = cnt > 0
+        want_rows = A.rows[lo:hi][nz]
+        want_len = cnt[nz].astype(np.int64)
+        want_h = href[nz]
+        got = {}
+        for mode in (cb.SPGEMM_PLAIN, cb.SPGEMM_LEARNED):
+            M, st = cb.dcsr.spgemm(controls, a, a, mode=mode, row_begin=lo, row_end=hi)
+            assert st["products"] == int(prod[lo:hi].sum())
+            rpt, rows, h, bad = _gpu_digests(controls, M)
+            what = f"block [{lo}, {hi}) mode {mode}"
+            assert bad == 0, f"{what}: {bad} rows with unsorted or repeated columns"
+            assert np.array_equal(rows, want_rows), f"{what}: row ids differ"
+            assert np.array_equal(np.diff(rpt), want_len), f"{what}: row sizes differ"
+            diff = np.flatnonzero(h != want_h)
+            assert diff.size == 0, f"{what}: {diff.size} rows differ, first row {rows[diff[0]]}"
+            got[mode] = (rpt, rows, M.nnz)
+            del M
+        assert np.array_equal(got[cb.SPGEMM_PLAIN][0], got[cb.SPGEMM_LEARNED][0])
+        total_products += int(prod[lo:hi].sum())
+        total_nnz += got[cb.SPGEMM_PLAIN][2]
+        checked_rows += int(nz.sum())
     assert total_products == int(prod.sum())
+    assert checked_rows > 500_000
     assert total_nnz > (1 << 32)  # the product does not fit one uint32 DCSR

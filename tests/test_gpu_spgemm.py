@@ -11,6 +11,7 @@ import numpy as np
 import pytest
 
 import clbool_amd as cb
+from clbool_amd import autotune as at
 import oracle
 from clbool_amd import generators as gen
 from helpers import assert_dcsr_equal, pairs_dcsr, upload
@@ -97,13 +98,17 @@ def test_hypersparse_2p24(controls):
     dict(hash_max_slots=16384, bitmap_min_products=1, bitmap_slot_ratio=1, tiny_max_products=0, bitmap_persistent=1),
     dict(hash_max_slots=1024, bitmap_min_products=4096, bitmap_slot_ratio=4, tiny_max_products=16,
          bitmap_persistent=1),
+    dict(hash_max_slots=16384, bitmap_min_products=1, bitmap_slot_ratio=1, tiny_max_products=0,
+         gather_combine=1),
+    dict(hash_max_slots=1024, bitmap_min_products=4096, bitmap_slot_ratio=4, tiny_max_products=16,
+         gather_combine=1),
 ])
 def test_every_kernel_path(controls, knobs):
     """Force the rows through each bin family (hash only, tiny, bitmap only)."""
     old = controls.tuning()
     t = controls.tuning()
     for k, v in knobs.items():
-        setattr(t, k, v)
+        at._set(t, k, v)
     controls.set_tuning(t)
     try:
         for scale, ncols in ((12, 1 << 12), (11, 1 << 22)):
