@@ -463,7 +463,7 @@ def roofline_of(cb, agg, workload):
     # between is not counted.  The compaction moves 4 B in and 4 B out per
     # column it copies.
     pass_, kind = divmod(fam, 4)
-    bins = {0: range(1, 2), 1: range(2, 10), 2: range(10, 32)}.get(kind, range(0))
+    bins = {0: range(1, 2), 1: range(2, 13), 2: range(13, 32)}.get(kind, range(0))
     writes = pass_ != 0
     rows_f = sum(agg["bin_rows"][b] for b in bins)
     bytes_total = sum(12 * agg["bin_a_nnz"][b] + 4 * agg["bin_products"][b] +

@@ -352,6 +352,14 @@ def _matrix_multiplication_hash(controls, a: matrix_dcsr, b: matrix_dcsr) -> mat
     return matrix_dcsr._from_c(m)
 
 
+def _matrix_multiplication(controls, a: matrix_dcsr, b: matrix_dcsr) -> matrix_dcsr:
+    """dcsr::matrix_multiplication (dcsr_matrix_multiplication.cpp:26-168): the
+    expand-sort-compress SpGEMM."""
+    m, ac, bc = _Dcsr(), a._c(), b._c()
+    _check(lib().clb_dcsr_spgemm_esc(controls.handle, C.byref(ac), C.byref(bc), C.byref(m)))
+    return matrix_dcsr._from_c(m)
+
+
 def _spgemm(controls, a: matrix_dcsr, b: matrix_dcsr, mode=SPGEMM_PLAIN, row_begin=0, row_end=0, timing=False):
     """The hash SpGEMM with a mode and an optional row block; returns (C, stats)."""
     m, ac, bc = _Dcsr(), a._c(), b._c()
@@ -475,6 +483,7 @@ def _csr_matrix_addition(controls, a: matrix_csr, b: matrix_csr) -> matrix_csr:
 
 
 dcsr = types.SimpleNamespace(matrix_multiplication_hash=_matrix_multiplication_hash,
+                             matrix_multiplication=_matrix_multiplication,
                              kronecker_product=_kronecker_product, spgemm=_spgemm, plan=SpgemmPlan,
                              addition=_dcsr_addition, digest=_digest,
                              transpose=_transpose, submatrix=_submatrix, reduce=_reduce)

@@ -153,6 +153,7 @@ SIGNATURES = [
     ("clb_dcsr_spgemm_hash", C.c_int32, [ctl_t, C.POINTER(Dcsr), C.POINTER(Dcsr), C.POINTER(Dcsr)]),
     ("clb_dcsr_spgemm", C.c_int32, [ctl_t, C.POINTER(Dcsr), C.POINTER(Dcsr), C.POINTER(SpgemmOpts),
                                     C.POINTER(Dcsr), C.POINTER(SpgemmStats)]),
+    ("clb_dcsr_spgemm_esc", C.c_int32, [ctl_t, C.POINTER(Dcsr), C.POINTER(Dcsr), C.POINTER(Dcsr)]),
     ("clb_spgemm_plan_create", C.c_int32, [ctl_t, C.POINTER(Dcsr), C.POINTER(Dcsr), C.c_uint32, C.c_uint64,
                                            C.POINTER(C.c_void_p)]),
     ("clb_spgemm_plan_info_get", C.c_int32, [C.c_void_p, C.POINTER(PlanInfo)]),

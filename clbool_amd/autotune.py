@@ -30,9 +30,10 @@ KNOBS = {
     "learned_margin_pct": [10, 25, 50],
     "bitmap_pred_caps": [0, 1],   # one pass: bitmap rows sized by the prediction (1) or the bound (0)
     "gather_combine": [0, 1],     # bitmap gather: one LDS atomic per word and wave (1) or per product (0)
+    "esc_max_products": [0, 2048, 8192],  # multi-window bitmap rows up to this many products sort in LDS
 }
 # knobs that live in clb_tuning.reserved[]
-RESERVED = {"bitmap_pred_caps": 0, "gather_combine": 1}
+RESERVED = {"bitmap_pred_caps": 0, "gather_combine": 1, "esc_max_products": 2}
 LEARNED_ONLY = ("learned_margin_pct", "bitmap_pred_caps")
 
 

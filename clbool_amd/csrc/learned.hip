@@ -299,6 +299,14 @@ __global__ __launch_bounds__(256) void k_compact_rows(const u32 *__restrict__ nn
     for (; t < n; t += 256) dst[t] = src[t];
 }
 
+void compact_rows(hipStream_t st, u32 nrows, const u32 *nnz_row, const u64 *off, const u32 *temp, const u64 *toff,
+                  const u32 *side, const u64 *soff, const u32 *key2, const u32 *isdense, const float *pred,
+                  u32 *cols, double *err) {
+    if (!nrows) return;
+    k_compact_rows<<<nrows, 256, 0, st>>>(nnz_row, off, temp, toff, side, soff, key2, isdense, pred, cols, err);
+    CLB_LAUNCHED("compact_rows");
+}
+
 // ---------------------------------------------------------------------------
 static void prepare_features(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &b, u32 rb, u32 nrows, Setup &s,
                              DevArray<float> &feat) {

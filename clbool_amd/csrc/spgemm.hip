@@ -262,6 +262,10 @@ BinCfg make_bincfg(const clb_controls_s *ctl, u32 ncols) {
     while (wb < 15 && (1ull << wb) < words) ++wb;
     c.sym_bits = wb;
     c.bm_bits = (wb == 15 && words <= (1ull << 15)) ? 14 : wb;
+    c.multi_slice = (u64)ncols > (32ull << c.bm_bits) ? 1u : 0u;
+    const u32 em = ctl->tuning.reserved[CLB_TUNE_ESC_MAX_PRODUCTS];
+    c.esc_max = em < (1u << ESC_MAX_BITS) ? em : (1u << ESC_MAX_BITS);
+    c.esc_only = 0;
     return c;
 }
 
@@ -516,8 +520,25 @@ void launch_bins(clb_controls_s *ctl, const Setup &s, const u32 *perm, const u32
         CLB_LAUNCHED("hash");
         t.end(fam(1), st);
     }
-    // BITMAP (all classes in one launch, largest rows first)
-    {
+    // ESC (rows of at most 2^11 / 2^12 / 2^13 products)
+    for (u32 bits = ESC_MIN_BITS; bits <= ESC_MAX_BITS; ++bits) {
+        const u32 b = BIN_ESC0 + (bits - ESC_MIN_BITS);
+        const u32 n = off[b + 1] - off[b];
+        if (!n) continue;
+        const u32 *pp = perm + off[b];
+        hipStream_t st = next();
+        t.begin(st);
+        switch (bits) {
+            case 11: DISPATCH3(k_esc, n, 256, st, 11, 256); break;
+            case 12: DISPATCH3(k_esc, n, 512, st, 12, 512); break;
+            case 13: DISPATCH3(k_esc, n, 1024, st, 13, 1024); break;
+        }
+        CLB_LAUNCHED("esc");
+        t.end(fam(1), st);
+    }
+    // BITMAP (all classes in one launch, largest rows first; the ESC SpGEMM
+    // sorts these rows in global memory instead)
+    if (!s.cfg.esc_only) {
         const u32 n = off[NBINS] - off[BIN_BITMAP0];
         if (n) {
             const u32 *pp = perm + off[BIN_BITMAP0];

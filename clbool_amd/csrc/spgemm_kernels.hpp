@@ -32,7 +32,10 @@ constexpr u32 BIN_TINY = 1;
 constexpr u32 BIN_HASH0 = 2;     // 2 + (log2 slots - 7)
 constexpr u32 HASH_MIN_BITS = 7;
 constexpr u32 HASH_MAX_BITS = 14;
-constexpr u32 BIN_BITMAP0 = 10;  // 10 + (31 - floor(log2 P)), clamped to 31
+constexpr u32 BIN_ESC0 = 10;     // 10 + (ceil(log2 P) - ESC_MIN_BITS): LDS expand-sort-compress rows
+constexpr u32 ESC_MIN_BITS = 11;
+constexpr u32 ESC_MAX_BITS = 13;
+constexpr u32 BIN_BITMAP0 = 13;  // 13 + (31 - floor(log2 P)), clamped to 31
 constexpr u32 BIG_ROW = 256;     // B rows at least this long are shared by the workgroup
 constexpr u32 LONG_ROW = 64;     // B rows at least this long are shared by a wave
 
@@ -84,20 +87,34 @@ struct BinCfg {
     u32 bitmap_ratio;
     u32 bm_bits;   // log2 of the bitmap window in 32-bit words (numeric and one-pass kernels)
     u32 sym_bits;  // the same for the symbolic kernel
+    // rows the hash rule sends to a bitmap that would need several column
+    // windows go to the LDS expand-sort-compress kernel instead when they
+    // have at most esc_max products (0: never)
+    u32 esc_max;
+    u32 multi_slice;  // ncols spans more than one bitmap window
+    // the ESC SpGEMM (dcsr::matrix_multiplication): tiny, LDS ESC up to
+    // 2^ESC_MAX_BITS products, longer rows to the global sort (BIN_BITMAP0)
+    u32 esc_only;
 };
 
 __device__ __forceinline__ u32 ceil_log2(u32 x) { return x <= 1 ? 0 : 32 - __clz(x - 1); }
 
 // P: products of the row; key: the number of distinct columns the table
 // must hold (min(P, ncols) before the symbolic pass, N after it).
+__device__ __forceinline__ u32 esc_bin(u32 P) {
+    const u32 b = ceil_log2(P);
+    return BIN_ESC0 + (b < ESC_MIN_BITS ? 0u : b - ESC_MIN_BITS);
+}
 __device__ __forceinline__ u32 classify(u32 P, u32 key, const BinCfg &c) {
     if (P == 0 || key == 0) return BIN_EMPTY;
     if (P <= c.tiny_max) return BIN_TINY;
+    if (c.esc_only) return P <= (1u << ESC_MAX_BITS) ? esc_bin(P) : BIN_BITMAP0;
     u32 bits = ceil_log2(2 * key);
     if (bits < HASH_MIN_BITS) bits = HASH_MIN_BITS;
     const bool bitmap = P >= c.bitmap_min || bits > c.hash_max_bits ||
                         (1ull << c.bm_bits) <= (u64)c.bitmap_ratio * (1ull << bits);
     if (!bitmap) return BIN_HASH0 + (bits - HASH_MIN_BITS);
+    if (c.multi_slice && P <= c.esc_max) return esc_bin(P);
     const u32 lp = 31 - __clz(P);
     const u32 b = BIN_BITMAP0 + (31 - lp);
     return b < NBINS ? b : NBINS - 1;
@@ -612,6 +629,69 @@ __global__ __launch_bounds__(WG) void k_hash(Mats g, const u32 *__restrict__ per
     if (MODE == NUM && n != lim && threadIdx.x == 0) atomicOr(o.overflow, 2u);
     u32 *out = o.cols + o.off[r];
     for (u32 t = threadIdx.x; t < w; t += WG) out[t] = sorted[t];
+}
+
+// ---------------------------------------------------------------------------
+// ESC rows (expand, sort, compress; the per-row scheme of bitonic_esc.cl:213-304):
+// the row's P <= 2^CAPB products are appended to an LDS array (one LDS
+// atomic per wave and call: the active lanes take consecutive slots by a
+// ballot), sorted by dev::lds_merge_sort (wave bitonic sorts of 64-blocks,
+// then merge-path rounds; the reference bitonic-sorts the whole array) and
+// compressed: each thread keeps the values of its contiguous stretch that
+// differ from their predecessor and writes them at its scanned position.
+template <int CAPB, int WG, int MODE>
+__global__ __launch_bounds__(WG) void k_esc(Mats g, const u32 *__restrict__ perm, Out o) {
+    constexpr u32 CAP = 1u << CAPB;
+    __shared__ __align__(16) u32 keys[CAP];
+    __shared__ __align__(16) u32 scratch[CAP];
+    __shared__ __align__(8) u32 big[BIG_WORDS(WG)];
+    __shared__ u32 red[WG / 64 + 1];
+    __shared__ u32 cnt;
+    const u32 r = perm[blockIdx.x];
+    const u32 i = g.row0 + r;
+    const u32 js = g.a_rpt[i], je = g.a_rpt[i + 1];
+    if (threadIdx.x == 0) cnt = 0;
+    __syncthreads();
+    auto append = [&](u32 col) {
+        const unsigned long long m = __ballot(1);
+        const u32 leader = (u32)__ffsll(m) - 1u;
+        u32 base = 0;
+        if ((u32)dev::lane() == leader) base = atomicAdd(&cnt, (u32)__popcll(m));
+        base = __shfl(base, (int)leader, dev::WAVE);
+        const u32 slot = base + dev::lanes_below(m);
+        if (slot < CAP) keys[slot] = col;
+    };
+    for_each_product<WG, 8, BIG_ROW>(g, js, je, big, append);
+    __syncthreads();
+    const u32 n = cnt < CAP ? cnt : CAP;  // the bin holds rows of at most CAP products
+    const u32 *sorted = dev::lds_merge_sort<WG>(keys, scratch, n);
+    const u32 per = (n + WG - 1) / WG;
+    const u32 t0 = threadIdx.x * per < n ? threadIdx.x * per : n;
+    const u32 t1 = t0 + per < n ? t0 + per : n;
+    u32 c = 0;
+    for (u32 t = t0; t < t1; ++t) c += (t == 0 || sorted[t] != sorted[t - 1]);
+    u32 total;
+    u32 pos = dev::block_excl_scan<u32, WG>(c, red, total);
+    if (MODE == SYM) {
+        if (threadIdx.x == 0) o.nnz_row[r] = total;
+        return;
+    }
+    const u32 lim = row_limit<MODE>(o, r);
+    if (MODE == ONE) {
+        if (total > lim) {
+            if (threadIdx.x == 0) {
+                o.nnz_row[r] = FAILED;
+                atomicAdd(o.overflow, 1u);
+            }
+            return;
+        }
+        if (threadIdx.x == 0) o.nnz_row[r] = total;
+    } else if (total != lim && threadIdx.x == 0) {
+        atomicOr(o.overflow, 2u);
+    }
+    u32 *out = o.cols + o.off[r];
+    for (u32 t = t0; t < t1; ++t)
+        if ((t == 0 || sorted[t] != sorted[t - 1]) && pos < lim) out[pos++] = sorted[t];
 }
 
 // ---------------------------------------------------------------------------

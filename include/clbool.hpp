@@ -10,9 +10,9 @@
 // clbool::buffer (reference-counted HIP allocations) instead of cl::Buffer,
 // Controls wraps a HIP device and stream instead of an OpenCL context and
 // queues, and kernel<> only checks names (kernels are compiled ahead of time).
-// Operations outside the SpGEMM path (dcsr::submatrix, reduce, transpose,
-// the ESC dcsr::matrix_multiplication, coo::kronecker_product) are not
-// provided.  Link with -lclbool_amd.
+// Provided beside the SpGEMM path: dcsr::matrix_multiplication (the
+// expand-sort-compress SpGEMM), dcsr::transpose, submatrix and reduce.  Not
+// provided: coo::kronecker_product.  Link with -lclbool_amd.
 #pragma once
 
 #include <algorithm>
@@ -460,6 +460,14 @@ inline void matrix_multiplication_hash(Controls &controls, matrix_dcsr &matrix_o
     clb_dcsr ca = a.c(), cbm = b.c(), out{};
     CLBOOL_CALL(clb_dcsr_spgemm_hash(controls.get(), &ca, &cbm, &out));
     matrix_out = matrix_dcsr::adopt(out);  // the output may alias an input
+}
+// dcsr::matrix_multiplication (src/dcsr/dcsr.hpp:28-31): the same product
+// by expand-sort-compress (bit-identical output).
+inline void matrix_multiplication(Controls &controls, matrix_dcsr &matrix_out, const matrix_dcsr &a,
+                                  const matrix_dcsr &b) {
+    clb_dcsr ca = a.c(), cbm = b.c(), out{};
+    CLBOOL_CALL(clb_dcsr_spgemm_esc(controls.get(), &ca, &cbm, &out));
+    matrix_out = matrix_dcsr::adopt(out);
 }
 // The same product through the learned one-pass mode (bit-identical output).
 inline void matrix_multiplication_learned(Controls &controls, matrix_dcsr &matrix_out, const matrix_dcsr &a,

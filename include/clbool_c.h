@@ -89,10 +89,17 @@ typedef struct clb_tuning {
                                       reserved[CLB_TUNE_GATHER_COMBINE]: nonzero:
                                       the bitmap gather ORs the bits that lanes
                                       of a wave set in one 32-bit word together
-                                      and issues one LDS atomic per word        */
+                                      and issues one LDS atomic per word
+                                      reserved[CLB_TUNE_ESC_MAX_PRODUCTS]: rows
+                                      bound for a bitmap whose columns span
+                                      several bitmap windows go to the LDS
+                                      expand-sort-compress kernel when they have
+                                      at most this many products (at most 8192;
+                                      0: never)                                 */
 } clb_tuning;
 #define CLB_TUNE_BITMAP_PRED_CAPS 0
 #define CLB_TUNE_GATHER_COMBINE   1
+#define CLB_TUNE_ESC_MAX_PRODUCTS 2
 
 #define CLB_SPGEMM_PLAIN   0u  /* two-pass: symbolic count, scan, numeric      */
 #define CLB_SPGEMM_LEARNED 1u  /* one pass sized by the learned predictor,
@@ -138,7 +145,7 @@ typedef struct clb_spgemm_stats {
 #define CLB_KP_ONEPASS  2u  /* learned / bound mode, the single sized pass */
 #define CLB_KP_FALLBACK 3u  /* learned mode, exact re-run of overflows */
 #define CLB_KK_TINY     0u
-#define CLB_KK_HASH     1u
+#define CLB_KK_HASH     1u  /* LDS hash tables and LDS expand-sort-compress rows */
 #define CLB_KK_BITMAP   2u
 #define CLB_KK_EMIT     3u  /* emission of dumped dense bitmap rows    */
 #define CLB_KF(pass, kind) ((pass) * 4u + (kind))
@@ -236,6 +243,12 @@ clb_status clb_dcsr_spgemm_hash(clb_controls ctl, const clb_dcsr *a, const clb_d
  * modes. */
 clb_status clb_dcsr_spgemm(clb_controls ctl, const clb_dcsr *a, const clb_dcsr *b, const clb_spgemm_opts *opts,
                            clb_dcsr *out, clb_spgemm_stats *stats);
+/* dcsr::matrix_multiplication (src/dcsr/dcsr_matrix_multiplication.cpp:26-168):
+ * the same product by expand-sort-compress (rows sorted in registers, in
+ * LDS or, above 2^13 products, as 64-bit keys in HBM), with no hash or
+ * bitmap accumulator: an independent cross-check of the hash path.  Same
+ * errors and empty-result shapes as clb_dcsr_spgemm_hash. */
+clb_status clb_dcsr_spgemm_esc(clb_controls ctl, const clb_dcsr *a, const clb_dcsr *b, clb_dcsr *out);
 
 /* ---- a product taken in row blocks (no counterpart in the reference: its
  *      matrix_multiplication_hash wraps around past 2^32 - 1 nonzeros,
