@@ -185,16 +185,10 @@ def train_predictor(cb, gen, ctl, log):
 
 
 def _fingerprint(ctl, m):
-    """Row structure in full plus column samples at both ends and the middle:
-    equal fingerprints of the two runs' matrices mean identical matrices
-    (structure) and identical sampled columns."""
-    if not m.nnz:
-        return (0,)
-    rpt = m.rpt_gpu.read(ctl, m.nzr + 1)
-    rows = m.rows_gpu.read(ctl, m.nzr)
-    k = min(m.nnz, 1 << 20)
-    cols = [m.cols_gpu.read(ctl, k, offset=4 * off) for off in (0, (m.nnz - k) // 2, m.nnz - k)]
-    return (m.nnz, m.nzr, hash(rpt.tobytes()), hash(rows.tobytes()), *(hash(c.tobytes()) for c in cols))
+    """(nnz, nzr, device-side digest of the whole matrix): equal fingerprints
+    of two results mean identical matrices (clb_dcsr_digest)."""
+    import clbool_amd as cb
+    return (int(m.nnz), int(m.nzr), cb.dcsr.digest(ctl, m))
 
 
 def bench_closure(cb, gen, ctl, max_iters, log, seed=0):
@@ -222,24 +216,26 @@ def bench_closure(cb, gen, ctl, max_iters, log, seed=0):
         del mats
         mats = [cb.matrix_dcsr_from_cpu(ctl, rpt, rows, cols, n) for rpt, rows, cols in dcsr]
         ctl.sync()
-        products, iters, steps_ms = 0, 0, []
+        products, iters, steps_ms, step_nnz = 0, 0, [], []
         t0 = time.perf_counter()
         for it in range(max_iters):
             ts = time.perf_counter()
             st = cb.closure_step(ctl, mats, triples, mode=mode, spgemm_mode=smode)
             steps_ms.append(1e3 * (time.perf_counter() - ts))
+            step_nnz.append(int(sum(m.nnz for m in mats)))
             products += st["products"]
             iters += 1
             if st["changed"] == 0:
                 break
         secs = time.perf_counter() - t0
-        finals[name] = [_fingerprint(ctl, m) for m in mats]
+        # every matrix compared in full by its device-side digest
+        finals[name] = [cb.dcsr.digest(ctl, m) for m in mats]
         out[name] = {"iterations": iters, "seconds": secs, "products": products,
                      "products_per_s": products / secs, "step_ms": [round(x, 2) for x in steps_ms],
-                     "fixed_point": st["changed"] == 0, "final_nnz": int(sum(m.nnz for m in mats))}
+                     "step_nnz": step_nnz, "fixed_point": st["changed"] == 0, "final_nnz": step_nnz[-1]}
         log(f"[bench] closure {name}: {iters} steps, {secs * 1e3:.1f} ms, {products / secs:.3e} products/s")
     out["identical_results"] = (finals["per_triple_plain"] == finals["batched_learned"] ==
-                                finals["batched_plain"])
+                                finals["batched_plain"])  # full-matrix digests of all 64 labels
     sec = {k: out[k]["seconds"] for k in finals}
     out["speedup"] = sec["per_triple_plain"] / sec["batched_learned"]  # learned + batched vs plain
     out["batching_speedup"] = sec["per_triple_plain"] / sec["batched_plain"]  # same SpGEMM mode
@@ -572,6 +568,8 @@ def main():
                     help="steps of the 64-label closure workload reported beside the main metric (0: skip)")
     ap.add_argument("--tuning-file", default=None,
                     help="JSON of knobs per mode (default profiles/tuning_<workload>.json when it exists)")
+    ap.add_argument("--set", action="append", default=[], metavar="KNOB=VALUE",
+                    help="override a tuning knob for every mode (after the tuning file)")
     ap.add_argument("--autotune", action="store_true", help="tune the knobs on the workload first")
     ap.add_argument("--save-tuning", default=None)
     ap.add_argument("--hypersparse", type=int, default=0,
@@ -628,7 +626,11 @@ def main():
     base_tuning = at.tuning_dict(ctl.tuning())
     tpath = args.tuning_file or os.path.join(ROOT, "profiles", f"tuning_{args.workload}.json")
     loaded = json.load(open(tpath)) if os.path.exists(tpath) else {}
-    tuned = {m: {**base_tuning, **loaded.get(m, {})} for m in modes}
+    overrides = {k: int(v) for k, v in (kv.split("=", 1) for kv in args.set)}
+    for k in overrides:
+        if k not in at.KNOBS:
+            raise SystemExit(f"unknown knob {k}; knobs: {sorted(at.KNOBS)}")
+    tuned = {m: {**base_tuning, **loaded.get(m, {}), **overrides} for m in modes}
     if loaded:
         log(f"[bench] knobs from {os.path.relpath(tpath, ROOT)}")
     block_bound = int(args.block_bound)
@@ -671,7 +673,7 @@ def main():
     for w in [w for w in args.also.split(",") if w and w != args.workload]:
         wt = os.path.join(ROOT, "profiles", f"tuning_{w}.json")
         wl = json.load(open(wt)) if os.path.exists(wt) else {}
-        tw = {m: {**base_tuning, **wl.get(m, {})} for m in modes}
+        tw = {m: {**base_tuning, **wl.get(m, {}), **overrides} for m in modes}
         res, _ = bench_product(cb, gen, ctl, oracle, w, modes, args.steps, args.warmup, tw, not args.no_validate,
                                block_bound, log)
         out[w] = res

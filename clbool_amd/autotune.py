@@ -31,10 +31,11 @@ KNOBS = {
     "bitmap_pred_caps": [0, 1],   # one pass: bitmap rows sized by the prediction (1) or the bound (0)
     "gather_combine": [0, 1],     # bitmap gather: one LDS atomic per word and wave (1) or per product (0)
     "esc_max_products": [0, 2048, 8192],  # multi-window bitmap rows up to this many products sort in LDS
+    "compact_tiles": [0, 1],      # one pass: compaction by output tiles (1) or one workgroup per row (0)
 }
 # knobs that live in clb_tuning.reserved[]
-RESERVED = {"bitmap_pred_caps": 0, "gather_combine": 1, "esc_max_products": 2}
-LEARNED_ONLY = ("learned_margin_pct", "bitmap_pred_caps")
+RESERVED = {"bitmap_pred_caps": 0, "gather_combine": 1, "esc_max_products": 2, "compact_tiles": 3}
+LEARNED_ONLY = ("learned_margin_pct", "bitmap_pred_caps", "compact_tiles")
 
 
 def _get(t, k):

@@ -148,8 +148,8 @@ void spgemm_esc(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &b, clb_d
         if (nnz) {
             clb_buffer_s *cols = buffer_new(ctl, 4ull * nnz);
             try {
-                compact_rows(ctl->stream, nrows, nnz_row.ptr, offs.ptr, temp.ptr, toff.ptr, u32p(L.cols), soff.ptr,
-                             from_side.ptr, nullptr, nullptr, u32p(cols), nullptr);
+                compact_rows(ctl->stream, nrows, nnz, nnz_row.ptr, offs.ptr, temp.ptr, toff.ptr, u32p(L.cols),
+                             soff.ptr, from_side.ptr, nullptr, nullptr, u32p(cols), nullptr, true);
                 assemble(ctl, s, a, nnz_row.ptr, offs.ptr, pos.ptr, nnz, nzr, cols, out);
                 sync(ctl);
             } catch (...) {

@@ -299,12 +299,113 @@ __global__ __launch_bounds__(256) void k_compact_rows(const u32 *__restrict__ nn
     for (; t < n; t += 256) dst[t] = src[t];
 }
 
-void compact_rows(hipStream_t st, u32 nrows, const u32 *nnz_row, const u64 *off, const u32 *temp, const u64 *toff,
-                  const u32 *side, const u64 *soff, const u32 *key2, const u32 *isdense, const float *pred,
-                  u32 *cols, double *err) {
+// The same copy cut by output position instead of by row: workgroup c writes
+// output columns [c * CC_TILE, (c + 1) * CC_TILE), so every workgroup moves
+// the same number of columns however the row sizes vary, and short rows do
+// not leave most of a workgroup idle.  The rows that overlap the tile are
+// found by binary search over the exact offsets (off[r] <= q < off[r + 1];
+// empty rows have off[r] == off[r + 1] and are never the last row with
+// off[r] <= q); their start and source are staged in LDS when at most
+// CC_ROWS rows overlap, and each thread finds its column's row by a binary
+// search there (or over off in HBM otherwise).
+constexpr u32 CC_WG = 256, CC_ITEMS = 16, CC_TILE = CC_WG * CC_ITEMS, CC_ROWS = 512;
+__device__ __forceinline__ u32 last_le(const u64 *__restrict__ a, u32 lo, u32 hi, u64 v) {
+    // last index i in [lo, hi) with a[i] <= v (a[lo] <= v)
+    while (hi - lo > 1) {
+        const u32 mid = lo + (hi - lo) / 2;
+        if (a[mid] <= v) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+__global__ __launch_bounds__(CC_WG) void k_compact_tiles(const u32 *__restrict__ nnz_row,
+                                                         const u64 *__restrict__ off, u32 nrows,
+                                                         const u32 *__restrict__ temp, const u64 *__restrict__ toff,
+                                                         const u32 *__restrict__ side, const u64 *__restrict__ soff,
+                                                         const u32 *__restrict__ key2,
+                                                         const u32 *__restrict__ isdense, u32 *__restrict__ cols) {
+    __shared__ u64 s_off[CC_ROWS + 1];
+    __shared__ u64 s_src[CC_ROWS];  // bit 63: from side; ~0: skip
+    __shared__ u32 s_r[2];
+    const u64 total = off[nrows];
+    const u64 q0 = (u64)blockIdx.x * CC_TILE;
+    const u64 q1 = q0 + CC_TILE < total ? q0 + CC_TILE : total;
+    if (threadIdx.x == 0) {
+        s_r[0] = last_le(off, 0, nrows, q0);
+        s_r[1] = last_le(off, 0, nrows, q1 - 1);
+    }
+    __syncthreads();
+    const u32 r0 = s_r[0], r1 = s_r[1];
+    const u32 m = r1 - r0 + 1;
+    auto source = [&](u32 r) -> u64 {
+        if (!nnz_row[r] || (isdense && isdense[r])) return ~0ull;
+        return (key2 && key2[r]) ? (soff[r] | (1ull << 63)) : toff[r];
+    };
+    const bool staged = m <= CC_ROWS;
+    if (staged) {
+        for (u32 j = threadIdx.x; j < m; j += CC_WG) {
+            s_off[j] = off[r0 + j];
+            s_src[j] = source(r0 + j);
+        }
+        __syncthreads();
+    }
+    for (u32 k = 0; k < CC_ITEMS; ++k) {
+        const u64 q = q0 + (u64)k * CC_WG + threadIdx.x;
+        if (q >= q1) break;
+        u64 start, src;
+        if (staged) {
+            u32 lo = 0, hi = m;
+            while (hi - lo > 1) {
+                const u32 mid = (lo + hi) / 2;
+                if (s_off[mid] <= q) lo = mid;
+                else hi = mid;
+            }
+            start = s_off[lo];
+            src = s_src[lo];
+        } else {
+            const u32 r = last_le(off, r0, r1 + 1, q);
+            start = off[r];
+            src = source(r);
+        }
+        if (src == ~0ull) continue;
+        const u64 d = q - start;
+        cols[q] = (src >> 63) ? side[(src & ~(1ull << 63)) + d] : temp[src + d];
+    }
+}
+
+// Mean-error numerator sum over rows of |log2(1 + pred) - log2(1 + n)|
+// (rows with output), one atomic per workgroup.
+__global__ __launch_bounds__(256) void k_pred_error(const u32 *__restrict__ nnz_row, const float *__restrict__ pred,
+                                                    u32 nrows, double *__restrict__ err) {
+    __shared__ double red[256 / 64 + 1];
+    const u32 r = blockIdx.x * 256 + threadIdx.x;
+    double e = 0.0;
+    if (r < nrows) {
+        const u32 n = nnz_row[r];
+        if (n) e = fabs(log2(1.0 + (double)pred[r]) - log2(1.0 + (double)n));
+    }
+    const double s = dev::block_sum<double, 256>(e, red);
+    if (threadIdx.x == 0 && s != 0.0) atomicAdd(err, s);
+}
+
+void compact_rows(hipStream_t st, u32 nrows, u64 nnz, const u32 *nnz_row, const u64 *off, const u32 *temp,
+                  const u64 *toff, const u32 *side, const u64 *soff, const u32 *key2, const u32 *isdense,
+                  const float *pred, u32 *cols, double *err, bool by_tiles) {
     if (!nrows) return;
-    k_compact_rows<<<nrows, 256, 0, st>>>(nnz_row, off, temp, toff, side, soff, key2, isdense, pred, cols, err);
-    CLB_LAUNCHED("compact_rows");
+    if (!by_tiles) {
+        k_compact_rows<<<nrows, 256, 0, st>>>(nnz_row, off, temp, toff, side, soff, key2, isdense, pred, cols, err);
+        CLB_LAUNCHED("compact_rows");
+        return;
+    }
+    if (pred && err) {
+        k_pred_error<<<div_up(nrows, 256), 256, 0, st>>>(nnz_row, pred, nrows, err);
+        CLB_LAUNCHED("pred_error");
+    }
+    if (nnz) {
+        k_compact_tiles<<<div_up(nnz, CC_TILE), CC_WG, 0, st>>>(nnz_row, off, nrows, temp, toff, side, soff, key2,
+                                                                isdense, cols);
+        CLB_LAUNCHED("compact_tiles");
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -495,10 +596,9 @@ void spgemm_learned(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &b, u
         fork_side(ctl);
         hipStream_t cs = side_stream(ctl, 0);
         lt.begin(cs);
-        k_compact_rows<<<nrows, 256, 0, cs>>>(nnz_row.ptr, offs.ptr, temp.ptr, toff.ptr, side.ptr, soff.ptr,
-                                              nfail ? key2.ptr : nullptr, dump.on ? dump.isdense.ptr : nullptr,
-                                              pred, u32p(cols), err.ptr);
-        CLB_LAUNCHED("compact_rows");
+        compact_rows(cs, nrows, nnz, nnz_row.ptr, offs.ptr, temp.ptr, toff.ptr, side.ptr, soff.ptr,
+                     nfail ? key2.ptr : nullptr, dump.on ? dump.isdense.ptr : nullptr, pred, u32p(cols), err.ptr,
+                     ctl->tuning.reserved[CLB_TUNE_COMPACT_TILES] != 0);
         lt.end((int)CLB_KF_COMPACT, cs);
         emit_dense(ctl, perm.ptr, h, dump, offs.ptr, u32p(cols), &lt, CLB_KP_ONEPASS, timing ? &st : nullptr);
         join_side(ctl);

@@ -449,15 +449,15 @@ LaunchTimer::~LaunchTimer() {
 // of perm.  numeric == false: write nnz_row; true: write columns.
 // One workgroup per row, or (tuning.bitmap_persistent) a persistent grid
 // that takes rows from a counter.
-template <int WBITS, int WG, int MODE>
-static void launch_bitmap(clb_controls_s *ctl, hipStream_t st, const Setup &s, const u32 *perm, u32 n,
-                          const Out &o) {
+template <int WBITS, int WG, int MODE, bool COMB>
+static void launch_bitmap_c(clb_controls_s *ctl, hipStream_t st, const Setup &s, const u32 *perm, u32 n,
+                            const Out &o) {
     if (!ctl->tuning.bitmap_persistent) {
-        k_bitmap<WBITS, WG, MODE><<<n, WG, 0, st>>>(s.g, perm, o);
+        k_bitmap<WBITS, WG, MODE, COMB><<<n, WG, 0, st>>>(s.g, perm, o);
         return;
     }
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_bitmap_persist<WBITS, WG, MODE>, WG, 0) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_bitmap_persist<WBITS, WG, MODE, COMB>, WG, 0) !=
             hipSuccess ||
         per_cu < 1) {
         (void)hipGetLastError();
@@ -466,7 +466,14 @@ static void launch_bitmap(clb_controls_s *ctl, hipStream_t st, const Setup &s, c
     const u64 fit = (u64)ctl->cu_count * (u64)per_cu;
     u32 *ctr = s.ctr.ptr + MODE;  // one counter per pass of the SpGEMM
     fill_u32_on(st, ctr, 1, 0u);
-    k_bitmap_persist<WBITS, WG, MODE><<<(u32)(n < fit ? n : fit), WG, 0, st>>>(s.g, perm, n, ctr, o);
+    k_bitmap_persist<WBITS, WG, MODE, COMB><<<(u32)(n < fit ? n : fit), WG, 0, st>>>(s.g, perm, n, ctr, o);
+}
+
+template <int WBITS, int WG, int MODE>
+static void launch_bitmap(clb_controls_s *ctl, hipStream_t st, const Setup &s, const u32 *perm, u32 n,
+                          const Out &o) {
+    if (s.g.combine) launch_bitmap_c<WBITS, WG, MODE, true>(ctl, st, s, perm, n, o);
+    else launch_bitmap_c<WBITS, WG, MODE, false>(ctl, st, s, perm, n, o);
 }
 
 void launch_bins(clb_controls_s *ctl, const Setup &s, const u32 *perm, const u32 off[NBINS + 1], int mode,

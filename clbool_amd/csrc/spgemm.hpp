@@ -121,13 +121,15 @@ void spgemm_learned(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &b, u
 // afterwards), capacities into size, predictions into pred (learned only).
 void plan_sizes_learned(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &b, Setup &s, u32 *size,
                         float *pred, bool bound_only);
-// Copies every row r with nnz_row[r] != 0 to cols + off[r]: from
-// side + soff[r] when key2[r] != 0, else from temp + toff[r]; rows with
-// isdense[r] are skipped (written elsewhere).  With pred, accumulates
-// |log2(1 + pred[r]) - log2(1 + nnz_row[r])| into *err.  (learned.hip)
-void compact_rows(hipStream_t st, u32 nrows, const u32 *nnz_row, const u64 *off, const u32 *temp, const u64 *toff,
-                  const u32 *side, const u64 *soff, const u32 *key2, const u32 *isdense, const float *pred,
-                  u32 *cols, double *err);
+// Copies every row r with nnz_row[r] != 0 to cols + off[r] (off: exact
+// offsets, off[nrows] = nnz): from side + soff[r] when key2[r] != 0, else
+// from temp + toff[r]; rows with isdense[r] are skipped (written elsewhere).
+// With pred, accumulates |log2(1 + pred[r]) - log2(1 + nnz_row[r])| into
+// *err.  by_tiles: workgroups take equal tiles of output columns instead of
+// one row each.  (learned.hip)
+void compact_rows(hipStream_t st, u32 nrows, u64 nnz, const u32 *nnz_row, const u64 *off, const u32 *temp,
+                  const u64 *toff, const u32 *side, const u64 *soff, const u32 *key2, const u32 *isdense,
+                  const float *pred, u32 *cols, double *err, bool by_tiles);
 // esc.hip: the expand-sort-compress SpGEMM (dcsr::matrix_multiplication)
 void spgemm_esc(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &b, clb_dcsr &out, clb_spgemm_stats &st);
 void spgemm_dispatch(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &b, const clb_spgemm_opts &o,

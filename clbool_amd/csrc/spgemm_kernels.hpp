@@ -698,7 +698,9 @@ __global__ __launch_bounds__(WG) void k_esc(Mats g, const u32 *__restrict__ perm
 // BITMAP rows: an LDS bitmap of 2^WBITS words covers 32 * 2^WBITS columns;
 // wider column ranges are processed in slices.
 // One row (position idx of the bitmap bins) of the bitmap kernel.
-template <int WBITS, int WG, int MODE>
+// COMB: full windows OR their words within the wave first (Mats::combine;
+// a separate instantiation, so the plain gather keeps its register budget).
+template <int WBITS, int WG, int MODE, bool COMB>
 __device__ __forceinline__ void bitmap_row(const Mats &g, const u32 *__restrict__ perm, u32 idx, const Out &o,
                                            u32 *bm, u32 *big, u32 *red) {
     constexpr u32 W = 1u << WBITS;
@@ -732,12 +734,16 @@ __device__ __forceinline__ void bitmap_row(const Mats &g, const u32 *__restrict_
         auto setdw = [&](u32 col) { dev::wave_or_sorted_bits(bm, (u32)((u64)col - base)); };
         constexpr int UBW = MODE == SYM ? 16 : 8;
         if (nslices == 1) {
-            if (g.combine) for_each_product<WG, UBW, LONG_ROW>(g, js, je, big, set1, setw);
+            if constexpr (COMB) for_each_product<WG, UBW, LONG_ROW>(g, js, je, big, set1, setw);
             else for_each_product<WG, UBW, LONG_ROW>(g, js, je, big, set1);
-        } else if (use_splits && g.combine) {  // only the slice's part of each B row is read
-            for_each_product<WG, UBW, LONG_ROW>(g, js, je, big, setd, setdw, s, sstep);
         } else {
-            for_each_product<WG, UBW, LONG_ROW>(g, js, je, big, setd, use_splits ? s : NOSLICE, sstep);
+            // only the slice's part of each B row is read (with the split table)
+            if constexpr (COMB) {
+                if (use_splits) for_each_product<WG, UBW, LONG_ROW>(g, js, je, big, setd, setdw, s, sstep);
+                else for_each_product<WG, UBW, LONG_ROW>(g, js, je, big, setd, NOSLICE, sstep);
+            } else {
+                for_each_product<WG, UBW, LONG_ROW>(g, js, je, big, setd, use_splits ? s : NOSLICE, sstep);
+            }
         }
         __syncthreads();
         if (MODE != SYM && WBITS >= 14 && o.dump.store) {
@@ -849,12 +855,12 @@ __device__ __forceinline__ void bitmap_row(const Mats &g, const u32 *__restrict_
 // BITMAP rows: one workgroup per row (a launch of one workgroup per row lets
 // the dispatcher balance rows of very different sizes over the CUs and
 // interleave the concurrent hash launches; a persistent grid did neither).
-template <int WBITS, int WG, int MODE>
+template <int WBITS, int WG, int MODE, bool COMB>
 __global__ __launch_bounds__(WG) void k_bitmap(Mats g, const u32 *__restrict__ perm, Out o) {
     __shared__ __align__(16) u32 bm[1u << WBITS];
     __shared__ __align__(8) u32 big[BIG_WORDS(WG)];
     __shared__ u32 red[WG / 64 + 1];
-    bitmap_row<WBITS, WG, MODE>(g, perm, blockIdx.x, o, bm, big, red);
+    bitmap_row<WBITS, WG, MODE, COMB>(g, perm, blockIdx.x, o, bm, big, red);
 }
 
 // The same rows, persistent: a grid of the workgroups that fit on the device
@@ -862,7 +868,7 @@ __global__ __launch_bounds__(WG) void k_bitmap(Mats g, const u32 *__restrict__ p
 // as soon as it finishes one -- dynamic balance over rows of very different
 // sizes without a dispatch per row.  The next row is claimed at the start of
 // the current one.
-template <int WBITS, int WG, int MODE>
+template <int WBITS, int WG, int MODE, bool COMB>
 __global__ __launch_bounds__(WG) void k_bitmap_persist(Mats g, const u32 *__restrict__ perm, u32 count,
                                                        u32 *__restrict__ ctr, Out o) {
     __shared__ __align__(16) u32 bm[1u << WBITS];
@@ -875,7 +881,7 @@ __global__ __launch_bounds__(WG) void k_bitmap_persist(Mats g, const u32 *__rest
     __syncthreads();
     while (idx < count) {
         if (threadIdx.x == 0) next = atomicAdd(ctr, 1u);
-        bitmap_row<WBITS, WG, MODE>(g, perm, idx, o, bm, big, red);
+        bitmap_row<WBITS, WG, MODE, COMB>(g, perm, idx, o, bm, big, red);
         __syncthreads();
         idx = next;
         __syncthreads();

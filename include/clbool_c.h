@@ -95,11 +95,16 @@ typedef struct clb_tuning {
                                       several bitmap windows go to the LDS
                                       expand-sort-compress kernel when they have
                                       at most this many products (at most 8192;
-                                      0: never)                                 */
+                                      0: never)
+                                      reserved[CLB_TUNE_COMPACT_TILES]: nonzero:
+                                      the one pass's compaction gives every
+                                      workgroup an equal tile of output columns
+                                      (0: one workgroup per row)                */
 } clb_tuning;
 #define CLB_TUNE_BITMAP_PRED_CAPS 0
 #define CLB_TUNE_GATHER_COMBINE   1
 #define CLB_TUNE_ESC_MAX_PRODUCTS 2
+#define CLB_TUNE_COMPACT_TILES    3
 
 #define CLB_SPGEMM_PLAIN   0u  /* two-pass: symbolic count, scan, numeric      */
 #define CLB_SPGEMM_LEARNED 1u  /* one pass sized by the learned predictor,
