@@ -185,6 +185,7 @@ SIGNATURES = [
     ("clb_mlp_adam", C.c_int32, [ctl_t, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_float, C.c_float,
                                  C.c_float, C.c_float, C.c_uint32]),
     ("clb_prefix_sum", C.c_int32, [ctl_t, buf_t, C.c_uint32, u32p]),
+    ("clb_prefix_sum_noalloc", C.c_int32, [ctl_t, buf_t, C.c_uint32, u32p]),
     ("clb_fill_with", C.c_int32, [ctl_t, buf_t, C.c_uint32, C.c_uint32]),
     ("clb_kernel_lookup", C.c_int32, [C.c_char_p, C.c_char_p]),
 ]

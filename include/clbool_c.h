@@ -389,6 +389,13 @@ clb_status clb_mlp_adam(clb_controls ctl, float *params, float *m, float *v, con
 /* prefix_sum (src/common/cl_operations.cpp:4-88): in-place exclusive scan of
  * n uint32 values; *total receives the sum. */
 clb_status clb_prefix_sum(clb_controls ctl, clb_buffer buf, uint32_t n, uint32_t *total);
+/* clb_prefix_sum without any allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * tile sums live in a fixed 256 KiB device array per GPU, so n is at most
+ * 2^28 (larger n: CLBOOL_INVALID_ARGUMENT).  Calls that overlap on one GPU
+ * share that array and race. */
+clb_status clb_prefix_sum_noalloc(clb_controls ctl, clb_buffer buf, uint32_t n, uint32_t *total);
 /* fill_with (src/common/cl_operations.cpp:90-100) */
 clb_status clb_fill_with(clb_controls ctl, clb_buffer buf, uint32_t n, uint32_t value);
 
