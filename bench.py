@@ -126,20 +126,18 @@ def _add_stats(agg, st):
 def run_steps(cb, ctl, a_dev, mode, n, timing, block_bound):
     """n full A*A products.  Each step plans its own row blocks (the mode's
     sizing pass over all rows of A: symbolic counts, predicted capacities or
-    bounds) and then runs every block; the planning is inside the timed
-    region.  Returns (seconds, statistics of the last step)."""
+    bounds) and then runs every block (clb_spgemm_plan_run_all); the planning
+    is inside the timed region.  Returns (seconds, statistics of the last
+    step)."""
     agg = None
     t0 = time.perf_counter()
     for _ in range(n):
         agg = _new_agg()
         plan = cb.dcsr.plan(ctl, a_dev, a_dev, mode=mode, block_bound=block_bound)
         agg["plan_ms"] = plan.info()["ms_plan"]
-        k = 0
-        while k < plan.nblocks:
-            C, st = plan.run(k, timing=timing)
-            del C
-            _add_stats(agg, st)
-            k += 1
+        # blocks run back to back (one-pass blocks overlap); each result is
+        # dropped as soon as it is complete
+        plan.run_all(lambda k, C, st: _add_stats(agg, st), timing=timing)
         info = plan.info()
         agg["blocks"], agg["replans"] = info["nblocks"], info["replans"]
         plan.close()
@@ -191,7 +189,7 @@ def _fingerprint(ctl, m):
     return (int(m.nnz), int(m.nzr), cb.dcsr.digest(ctl, m))
 
 
-def bench_closure(cb, gen, ctl, max_iters, log, seed=0):
+def bench_closure(cb, gen, ctl, max_iters, log, seed=0, legs=None, warm=True):
     """The 64-label closure on 2^18 vertices: 32 seeded triples, X |= Y * Z to
     a fixed point or max_iters steps.  Per-triple plain SpGEMMs (the
     reference's call pattern), batched plain SpGEMMs (batching alone) and
@@ -205,15 +203,17 @@ def bench_closure(cb, gen, ctl, max_iters, log, seed=0):
     dcsr = [gen.to_dcsr(r, c) for r, c in labs]
     out = {"labels": 64, "vertices": n, "triples": 32, "label_nnz": int(sum(d[2].size for d in dcsr))}
     finals = {}
-    for name, mode, smode in (("per_triple_plain", cb.CLOSURE_PER_TRIPLE, cb.SPGEMM_PLAIN),
-                              ("batched_plain", cb.CLOSURE_BATCHED, cb.SPGEMM_PLAIN),
-                              ("batched_learned", cb.CLOSURE_BATCHED, cb.SPGEMM_LEARNED)):
+    all_legs = (("per_triple_plain", cb.CLOSURE_PER_TRIPLE, cb.SPGEMM_PLAIN),
+                ("batched_plain", cb.CLOSURE_BATCHED, cb.SPGEMM_PLAIN),
+                ("batched_learned", cb.CLOSURE_BATCHED, cb.SPGEMM_LEARNED))
+    for name, mode, smode in [x for x in all_legs if legs is None or x[0] in legs]:
         ctl.trim()
-        mats = [cb.matrix_dcsr_from_cpu(ctl, rpt, rows, cols, n) for rpt, rows, cols in dcsr]
-        for it in range(max_iters):  # warm-up
-            if cb.closure_step(ctl, mats, triples, mode=mode, spgemm_mode=smode)["changed"] == 0:
-                break
-        del mats
+        if warm:
+            mats = [cb.matrix_dcsr_from_cpu(ctl, rpt, rows, cols, n) for rpt, rows, cols in dcsr]
+            for it in range(max_iters):  # warm-up
+                if cb.closure_step(ctl, mats, triples, mode=mode, spgemm_mode=smode)["changed"] == 0:
+                    break
+            del mats
         mats = [cb.matrix_dcsr_from_cpu(ctl, rpt, rows, cols, n) for rpt, rows, cols in dcsr]
         ctl.sync()
         products, iters, steps_ms, step_nnz = 0, 0, [], []
@@ -234,13 +234,17 @@ def bench_closure(cb, gen, ctl, max_iters, log, seed=0):
                      "products_per_s": products / secs, "step_ms": [round(x, 2) for x in steps_ms],
                      "step_nnz": step_nnz, "fixed_point": st["changed"] == 0, "final_nnz": step_nnz[-1]}
         log(f"[bench] closure {name}: {iters} steps, {secs * 1e3:.1f} ms, {products / secs:.3e} products/s")
-    out["identical_results"] = (finals["per_triple_plain"] == finals["batched_learned"] ==
-                                finals["batched_plain"])  # full-matrix digests of all 64 labels
+    # full-matrix digests of all 64 labels
+    out["identical_results"] = len({tuple(v) for v in finals.values()}) == 1
+    out["warm"] = warm
     sec = {k: out[k]["seconds"] for k in finals}
-    out["speedup"] = sec["per_triple_plain"] / sec["batched_learned"]  # learned + batched vs plain
-    out["batching_speedup"] = sec["per_triple_plain"] / sec["batched_plain"]  # same SpGEMM mode
-    out["learned_speedup_batched"] = sec["batched_plain"] / sec["batched_learned"]
-    out["fixed_point"] = out["batched_learned"]["fixed_point"]
+    if "per_triple_plain" in sec and "batched_learned" in sec:
+        out["speedup"] = sec["per_triple_plain"] / sec["batched_learned"]  # learned + batched vs plain
+    if "per_triple_plain" in sec and "batched_plain" in sec:
+        out["batching_speedup"] = sec["per_triple_plain"] / sec["batched_plain"]  # same SpGEMM mode
+    if "batched_plain" in sec and "batched_learned" in sec:
+        out["learned_speedup_batched"] = sec["batched_plain"] / sec["batched_learned"]
+    out["fixed_point"] = all(out[k]["fixed_point"] for k in finals)
     return out
 
 
@@ -576,6 +580,9 @@ def main():
                     help="also run the hypersparse learned-sizing workload, training on this many instances")
     ap.add_argument("--hypersparse-only", type=int, default=0,
                     help="run only the hypersparse workload, training on this many instances")
+    ap.add_argument("--closure-only", type=int, default=0,
+                    help="run only the 64-label closure, up to this many steps or its fixed point, per-triple "
+                         "plain against batched learned, without the warm-up run")
     ap.add_argument("--c5-only", action="store_true",
                     help="run only the automaton (x) R-MAT 22/32 workload: the Kronecker in automaton-row "
                          "blocks and a bounded sample of one K * K closure step")
@@ -602,6 +609,15 @@ def main():
         from clbool_amd import generators as gen
         ctl = cb.create_controls()
         print(json.dumps({"c5": bench_c5(cb, gen, ctl, log)}), flush=True)
+        return
+    if args.closure_only:
+        import clbool_amd as cb
+        from clbool_amd import generators as gen
+        ctl = cb.create_controls()
+        train_predictor(cb, gen, ctl, log)
+        print(json.dumps({"closure": bench_closure(cb, gen, ctl, args.closure_only, log,
+                                                   legs=("per_triple_plain", "batched_learned"), warm=False)}),
+              flush=True)
         return
     if args.hypersparse_only:
         import clbool_amd as cb

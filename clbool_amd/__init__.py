@@ -410,6 +410,24 @@ class SpgemmPlan:
             yield self.run(k)
             k += 1
 
+    def run_all(self, fn, timing=False):
+        """Every block in order: fn(k, C_block, stats) as soon as block k is
+        complete (a true return value stops the run).  In the one-pass modes
+        the next block's passes overlap this block's last phase."""
+        err = []
+
+        def cb(user, k, blk, st):
+            try:
+                return 1 if fn(k, matrix_dcsr._from_c(blk.contents), st.contents.as_dict()) else 0
+            except BaseException as e:  # reported after the call returns
+                err.append(e)
+                return 1
+        c_fn = _capi.BLOCK_FN(cb)
+        rc = lib().clb_spgemm_plan_run_all(self.controls.handle, self.handle, int(timing), c_fn, None)
+        if err:
+            raise err[0]
+        _check(rc)
+
     def close(self):
         if self.handle:
             lib().clb_spgemm_plan_release(self.handle)

@@ -77,6 +77,9 @@ KERNEL_FAMILIES = [f"{p}_{k}" for p in KERNEL_PASSES for k in KERNEL_KINDS]
 KERNEL_FAMILIES[15] = "onepass_compact"
 
 
+BLOCK_FN = C.CFUNCTYPE(C.c_int32, C.c_void_p, C.c_uint32, C.POINTER(Dcsr), C.POINTER(SpgemmStats))
+
+
 class PlanInfo(C.Structure):
     _fields_ = [("mode", C.c_uint32), ("nblocks", C.c_uint32), ("replans", C.c_uint32), ("reserved", C.c_uint32),
                 ("products", C.c_uint64), ("planned_nnz", C.c_uint64), ("ms_plan", C.c_double)]
@@ -160,6 +163,7 @@ SIGNATURES = [
     ("clb_spgemm_plan_block", C.c_int32, [C.c_void_p, C.c_uint32, u32p, u32p]),
     ("clb_spgemm_plan_run", C.c_int32, [ctl_t, C.c_void_p, C.c_uint32, C.c_uint32, C.POINTER(Dcsr),
                                         C.POINTER(SpgemmStats)]),
+    ("clb_spgemm_plan_run_all", C.c_int32, [ctl_t, C.c_void_p, C.c_uint32, BLOCK_FN, C.c_void_p]),
     ("clb_spgemm_plan_release", C.c_int32, [C.c_void_p]),
     ("clb_dcsr_kronecker", C.c_int32, [ctl_t, C.POINTER(Dcsr), C.POINTER(Dcsr), C.POINTER(Dcsr)]),
     ("clb_dcsr_transpose", C.c_int32, [ctl_t, C.POINTER(Dcsr), C.POINTER(Dcsr)]),

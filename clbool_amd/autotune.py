@@ -33,17 +33,27 @@ KNOBS = {
     "esc_max_products": [0, 2048, 8192],  # multi-window bitmap rows up to this many products sort in LDS
     "compact_tiles": [0, 1],      # one pass: compaction by output tiles (1) or one workgroup per row (0)
 }
-# knobs that live in clb_tuning.reserved[]
+# workgroup shape of each LDS hash table size: default (0) or twice the threads (1)
+for _b in range(7, 14):
+    KNOBS[f"hash_wide_{_b}"] = [0, 1]
+# knobs that live in clb_tuning.reserved[] (whole words, or one bit of a word)
 RESERVED = {"bitmap_pred_caps": 0, "gather_combine": 1, "esc_max_products": 2, "compact_tiles": 3}
+RESERVED_BITS = {f"hash_wide_{b}": (4, b - 7) for b in range(7, 14)}
 LEARNED_ONLY = ("learned_margin_pct", "bitmap_pred_caps", "compact_tiles")
 
 
 def _get(t, k):
+    if k in RESERVED_BITS:
+        w, b = RESERVED_BITS[k]
+        return (t.reserved[w] >> b) & 1
     return t.reserved[RESERVED[k]] if k in RESERVED else getattr(t, k)
 
 
 def _set(t, k, v):
-    if k in RESERVED:
+    if k in RESERVED_BITS:
+        w, b = RESERVED_BITS[k]
+        t.reserved[w] = (t.reserved[w] & ~(1 << b)) | ((int(v) & 1) << b)
+    elif k in RESERVED:
         t.reserved[RESERVED[k]] = v
     else:
         setattr(t, k, v)

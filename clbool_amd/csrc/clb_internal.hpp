@@ -108,6 +108,11 @@ struct clb_controls_s {
     hipStream_t side[NSIDE] = {};  // default-priority side streams for per-bin launches
     hipEvent_t ev_fork = nullptr;
     hipEvent_t ev_join[NSIDE] = {};
+    // default-priority streams for the last phase of a one-pass block
+    // (compaction, dense-row emission); a pipelined plan lets them run under
+    // the next block's passes
+    static constexpr int NTAIL = 2;
+    hipStream_t tail[NTAIL] = {};
     std::mutex mu;
 };
 

@@ -308,17 +308,13 @@ clb_status clb_closure_step(clb_controls ctl, clb_dcsr *mats, uint32_t nmats, co
                         if (hb[k] > st.max_block_bound) st.max_block_bound = hb[k];
                     }
                     DevArray<u32> sp(ctl, 2ull * (ntriples + 1));
-                    for (u32 k = 0; k < nb; ++k) {
-                        if (hc[k + 1] <= hc[k]) continue;
-                        clb_spgemm_opts bo = so;
-                        bo.row_begin = hc[k];
-                        bo.row_end = hc[k + 1];
-                        clb_spgemm_stats ss;
-                        clb_dcsr T{};
-                        spg::spgemm_dispatch(ctl, Y, Z, bo, T, ss);
+                    auto absorb = [&](clb_dcsr &T, const clb_spgemm_stats &ss) {
                         st.products += ss.products;
                         ++st.spgemm_calls;
-                        if (!T.nnz) continue;
+                        if (!T.nnz) {
+                            clb_dcsr_release(&T);
+                            return;
+                        }
                         prods.push_back(T);
                         // where each product block t starts in T
                         k_row_splits<<<div_up(ntriples + 1, 256), 256, 0, ctl->stream>>>(
@@ -327,7 +323,43 @@ clb_status clb_closure_step(clb_controls ctl, clb_dcsr *mats, uint32_t nmats, co
                         std::vector<u32> hs(2 * (ntriples + 1));
                         readback(ctl, hs.data(), sp.ptr, 4ull * hs.size());
                         splits.push_back(hs);
+                    };
+                    const bool one_pass = so.mode == CLB_SPGEMM_LEARNED || so.mode == CLB_SPGEMM_BOUND;
+                    // one-pass blocks overlap: block k + 1's passes start while
+                    // block k's compaction and dense-row emission still run
+                    // (when device memory allows two blocks at once)
+                    spg::OnePassPtr pend;
+                    auto finish_pending = [&]() {
+                        clb_dcsr T{};
+                        clb_spgemm_stats ss;
+                        spg::one_pass_finish(*pend, T, ss);
+                        pend.reset();
+                        absorb(T, ss);
+                    };
+                    for (u32 k = 0; k < nb; ++k) {
+                        if (hc[k + 1] <= hc[k]) continue;
+                        if (!one_pass) {
+                            clb_spgemm_opts bo = so;
+                            bo.row_begin = hc[k];
+                            bo.row_end = hc[k + 1];
+                            clb_spgemm_stats ss;
+                            clb_dcsr T{};
+                            spg::spgemm_dispatch(ctl, Y, Z, bo, T, ss);
+                            absorb(T, ss);
+                            continue;
+                        }
+                        if (pend) {
+                            size_t free_b = 0, total_b = 0;
+                            if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) (void)hipGetLastError();
+                            if (free_b + pool_for(ctl->device).cached_bytes() < 3 * spg::one_pass_bytes(*pend))
+                                finish_pending();
+                        }
+                        spg::OnePassPtr cur = spg::one_pass_start(ctl, Y, Z, hc[k], hc[k + 1], false,
+                                                                  so.mode == CLB_SPGEMM_BOUND, nullptr);
+                        if (pend) finish_pending();
+                        pend = std::move(cur);
                     }
+                    if (pend) finish_pending();
                 }
             } catch (...) {
                 release_all();

@@ -469,15 +469,50 @@ void plan_sizes_learned(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &
     s.cfg = learned_bincfg(ctl, b.ncols);
 }
 
-void spgemm_learned(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &b, u32 rb, u32 re, bool timing,
-                    clb_dcsr &out, clb_spgemm_stats &st, bool bound_only, const PlanRows *pr) {
-    const u32 nrows = re - rb;
+// One block of the one pass, alive until its columns are written.  Its last
+// phase (the compaction of the temporary slots and the dense-row emission)
+// runs on the controls' tail streams, so a pipelined plan can start the next
+// block's passes before it ends; every buffer it reads stays here until then.
+struct OnePass {
+    clb_controls_s *ctl = nullptr;
     Setup s;
     LaunchTimer lt;
-    lt.on = timing;
-    auto t0 = std::chrono::steady_clock::now();
-    DevArray<u32> cap_own, nnz_row(ctl, nrows);
+    DevArray<u32> cap_own, nnz_row, perm, key2, perm2, side, pos, temp;
     DevArray<float> pred_own;
+    DevArray<u64> toff, soff, offs, gpre;
+    DevArray<double> err;
+    DumpBufs dump;
+    BinHost h{};
+    clb_dcsr out{};
+    clb_spgemm_stats st{};
+    hipEvent_t joined = nullptr, done = nullptr;
+    bool tail = false, timing = false, replan = false;
+    std::chrono::steady_clock::time_point t0;
+    ~OnePass() {
+        if (tail)
+            for (int k = 0; k < clb_controls_s::NTAIL; ++k) (void)hipStreamSynchronize(ctl->tail[k]);
+        if (joined) (void)hipEventDestroy(joined);
+        if (done) (void)hipEventDestroy(done);
+        clb_dcsr_release(&out);  // nothing left when the result was handed over
+    }
+};
+
+void OnePassDeleter::operator()(OnePass *p) const { delete p; }
+
+OnePassPtr one_pass_start(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &b, u32 rb, u32 re, bool timing,
+                          bool bound_only, const PlanRows *pr) {
+    OnePassPtr P(new OnePass());
+    OnePass &p = *P;
+    p.ctl = ctl;
+    p.timing = timing;
+    p.lt.on = timing;
+    p.t0 = std::chrono::steady_clock::now();
+    p.out.nrows = a.nrows;
+    p.out.ncols = b.ncols;
+    clb_spgemm_stats &st = p.st;
+    Setup &s = p.s;
+    const u32 nrows = re - rb;
+    p.nnz_row.reset(ctl, nrows);
     u32 *cap = nullptr;
     const float *pred = nullptr;
     if (pr) {
@@ -490,84 +525,72 @@ void spgemm_learned(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &b, u
     } else {
         s.nrows = nrows;
         s.row0 = rb;
-        cap_own.reset(ctl, nrows);
-        if (!bound_only) pred_own.reset(ctl, nrows);
-        learned_caps(ctl, a, b, s, bound_only, cap_own.ptr, pred_own.ptr);
-        cap = cap_own.ptr;
-        pred = pred_own.ptr;
+        p.cap_own.reset(ctl, nrows);
+        if (!bound_only) p.pred_own.reset(ctl, nrows);
+        learned_caps(ctl, a, b, s, bound_only, p.cap_own.ptr, p.pred_own.ptr);
+        cap = p.cap_own.ptr;
+        pred = p.pred_own.ptr;
     }
     s.cfg = learned_bincfg(ctl, b.ncols);
-    DevArray<u32> perm;
     BinBuf bb;
-    BinHost h;
-    bin_rows(ctl, s, cap, nnz_row.ptr, perm, bb);
-    DevArray<u64> toff(ctl, (u64)nrows + 1);
-    exclusive_scan_u32_to_u64(ctl, cap, toff.ptr, nrows, s.scal.ptr + 4);
-    read_bins(ctl, bb, h);
+    bin_rows(ctl, s, cap, p.nnz_row.ptr, p.perm, bb);
+    p.toff.reset(ctl, (u64)nrows + 1);
+    exclusive_scan_u32_to_u64(ctl, cap, p.toff.ptr, nrows, s.scal.ptr + 4);
+    read_bins(ctl, bb, p.h);
     u64 sc[8];
     readback(ctl, sc, s.scal.ptr, sizeof(sc));
     st.products = pr ? pr->products : sc[0];
     st.rows = nrows;
     st.n_bins = NBINS;
     st.capacity = sc[4];
-    if (st.products == 0) {
-        out.nrows = a.nrows;
-        out.ncols = b.ncols;
-        return;
-    }
-    DevArray<u32> temp(ctl, sc[4] ? sc[4] : 1);
+    if (st.products == 0) return P;
+    p.temp.reset(ctl, sc[4] ? sc[4] : 1);
     u32 *fails = reinterpret_cast<u32 *>(s.scal.ptr + 5);
-    Out o{nnz_row.ptr, toff.ptr, temp.ptr, cap, fails, Dump{}, ONE_PASS_PROBES};
-    DumpBufs dump;
-    setup_dump(ctl, s, h, dump, o, true);
-    ensure_splits(ctl, s, h);
-    launch_bins(ctl, s, perm.ptr, h.off, ONE, o, &lt, CLB_KP_ONEPASS);
+    Out o{p.nnz_row.ptr, p.toff.ptr, p.temp.ptr, cap, fails, Dump{}, ONE_PASS_PROBES};
+    setup_dump(ctl, s, p.h, p.dump, o, true, p.perm.ptr, cap);
+    ensure_splits(ctl, s, p.h);
+    launch_bins(ctl, s, p.perm.ptr, p.h.off, ONE, o, &p.lt, CLB_KP_ONEPASS);
     u32 nfail = 0;
     readback(ctl, &nfail, fails, 4);
     st.fallback_rows = nfail;
     if (timing) {  // per-bin statistics of the one-pass launch (for its roofline)
         u64 outs[NBINS];
-        sum_by_bin(ctl, perm.ptr, h, nrows, nnz_row.ptr, outs);
+        sum_by_bin(ctl, p.perm.ptr, p.h, nrows, p.nnz_row.ptr, outs);
         for (u32 k = 0; k < NBINS; ++k) {
-            st.bin_rows[k] = h.off[k + 1] - h.off[k];
-            st.bin_products[k] = h.products[k];
-            st.bin_a_nnz[k] = h.a_nnz[k];
+            st.bin_rows[k] = p.h.off[k + 1] - p.h.off[k];
+            st.bin_products[k] = p.h.products[k];
+            st.bin_a_nnz[k] = p.h.a_nnz[k];
             st.bin_out_nnz[k] = outs[k];
         }
     }
     // exact re-run of the rows that overflowed their predicted capacity
-    DevArray<u32> key2, perm2, side;
-    DevArray<u64> soff;
     if (nfail) {
-        key2.reset(ctl, nrows);
-        k_failed_keys<<<div_up(nrows, 256), 256, 0, ctl->stream>>>(nnz_row.ptr, s.P, nrows, b.ncols,
-                                                                    key2.ptr);
+        p.key2.reset(ctl, nrows);
+        k_failed_keys<<<div_up(nrows, 256), 256, 0, ctl->stream>>>(p.nnz_row.ptr, s.P, nrows, b.ncols,
+                                                                    p.key2.ptr);
         CLB_LAUNCHED("failed_keys");
-        Setup &sx = s;
-        const BinCfg keep = sx.cfg;
-        sx.cfg = make_bincfg(ctl, b.ncols);  // exact sizing as in the plain symbolic pass
+        const BinCfg keep = s.cfg;
+        s.cfg = make_bincfg(ctl, b.ncols);  // exact sizing as in the plain symbolic pass
         BinBuf bb2;
         BinHost h2;
-        bin_rows(ctl, sx, key2.ptr, nullptr, perm2, bb2);
-        soff.reset(ctl, (u64)nrows + 1);
-        exclusive_scan_u32_to_u64(ctl, key2.ptr, soff.ptr, nrows, s.scal.ptr + 6);
+        bin_rows(ctl, s, p.key2.ptr, nullptr, p.perm2, bb2);
+        p.soff.reset(ctl, (u64)nrows + 1);
+        exclusive_scan_u32_to_u64(ctl, p.key2.ptr, p.soff.ptr, nrows, s.scal.ptr + 6);
         read_bins(ctl, bb2, h2);
         u64 side_total = 0;
         readback(ctl, &side_total, s.scal.ptr + 6, 8);
-        side.reset(ctl, side_total ? side_total : 1);
+        p.side.reset(ctl, side_total ? side_total : 1);
         u32 *fails2 = reinterpret_cast<u32 *>(s.scal.ptr + 7);
-        Out o2{nnz_row.ptr, soff.ptr, side.ptr, key2.ptr, fails2, Dump{}, 0};  // exact: full probing
-        ensure_splits(ctl, sx, h2);
-        launch_bins(ctl, sx, perm2.ptr, h2.off, ONE, o2, &lt, CLB_KP_FALLBACK);
-        sx.cfg = keep;
+        Out o2{p.nnz_row.ptr, p.soff.ptr, p.side.ptr, p.key2.ptr, fails2, Dump{}, 0};  // exact: full probing
+        ensure_splits(ctl, s, h2);
+        launch_bins(ctl, s, p.perm2.ptr, h2.off, ONE, o2, &p.lt, CLB_KP_FALLBACK);
+        s.cfg = keep;
         u32 nfail2 = 0;
         readback(ctl, &nfail2, fails2, 4);
         if (nfail2) raise(RUN_KERNEL_FAILURE, "exact re-run overflowed its upper-bound capacity");
     }
-    // exact offsets and compaction
-    DevArray<u64> offs;
-    DevArray<u32> pos;
-    scan_sizes(ctl, s, nnz_row.ptr, offs, pos);
+    // exact offsets
+    scan_sizes(ctl, s, p.nnz_row.ptr, p.offs, p.pos);
     u64 sc2[3];
     readback(ctl, sc2, s.scal.ptr, sizeof(sc2));
     const u64 nnz = sc2[1];
@@ -575,49 +598,78 @@ void spgemm_learned(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &b, u
     if (pr && nnz > pr->bound) {
         // the block's result exceeds the plan's bound (rows grew past their
         // capacities): hand the exact row sizes back for a finer cut
-        CLB_HIP(hipMemcpyAsync(pr->size, nnz_row.ptr, 4ull * nrows, hipMemcpyDeviceToDevice, ctl->stream),
+        CLB_HIP(hipMemcpyAsync(pr->size, p.nnz_row.ptr, 4ull * nrows, hipMemcpyDeviceToDevice, ctl->stream),
                 COPY_BUFFER_ERROR);
         sync(ctl);
         *pr->replan = true;
-        out.nrows = a.nrows;
-        out.ncols = b.ncols;
-        return;
+        p.replan = true;
+        return P;
     }
-    if (nnz > 0xFFFFFFFFull) raise(INVALID_VALUE, "result has more nonzeros than uint32 indices allow; multiply by row blocks");
-    out.nrows = a.nrows;
-    out.ncols = b.ncols;
-    DevArray<double> err(ctl, 1);
-    fill_u64(ctl, reinterpret_cast<u64 *>(err.ptr), 1, 0ull);
-    if (nnz == 0) return;
-    clb_buffer_s *cols = buffer_new(ctl, 4ull * nnz);
-    try {
-        // the compaction (rows from the temporary slots) and the dense-row
-        // emission write disjoint rows: they run side by side
-        fork_side(ctl);
-        hipStream_t cs = side_stream(ctl, 0);
-        lt.begin(cs);
-        compact_rows(cs, nrows, nnz, nnz_row.ptr, offs.ptr, temp.ptr, toff.ptr, side.ptr, soff.ptr,
-                     nfail ? key2.ptr : nullptr, dump.on ? dump.isdense.ptr : nullptr, pred, u32p(cols), err.ptr,
-                     ctl->tuning.reserved[CLB_TUNE_COMPACT_TILES] != 0);
-        lt.end((int)CLB_KF_COMPACT, cs);
-        emit_dense(ctl, perm.ptr, h, dump, offs.ptr, u32p(cols), &lt, CLB_KP_ONEPASS, timing ? &st : nullptr);
-        join_side(ctl);
-        assemble(ctl, s, a, nnz_row.ptr, offs.ptr, pos.ptr, nnz, nzr, cols, out);
-        double e = 0;
-        readback(ctl, &e, err.ptr, 8);
-        st.pred_abs_log2_err = nzr ? e / nzr : 0.0;
-    } catch (...) {
-        (void)hipStreamSynchronize(ctl->stream);
-        for (int k = 0; k < clb_controls_s::NSIDE; ++k) (void)hipStreamSynchronize(ctl->side[k]);
-        clb_dcsr_release(&out);
-        if (!out.cols) buffer_release(cols);
-        throw;
-    }
+    if (nnz > 0xFFFFFFFFull)
+        raise(INVALID_VALUE, "result has more nonzeros than uint32 indices allow; multiply by row blocks");
     st.nnz = nnz;
     st.nzr = nzr;
-    lt.collect(st);
-    if (timing)
-        st.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (nnz == 0) return P;
+    p.err.reset(ctl, 1);
+    fill_u64(ctl, reinterpret_cast<u64 *>(p.err.ptr), 1, 0ull);
+    clb_buffer_s *cols = buffer_new(ctl, 4ull * nnz);
+    p.out.cols = cols;  // owned by p.out from here
+    const u32 ndense = emit_prepare(ctl, p.dump, p.gpre, timing ? &st : nullptr);
+    assemble(ctl, s, a, p.nnz_row.ptr, p.offs.ptr, p.pos.ptr, nnz, nzr, cols, p.out);
+    // the tail: compaction of the temporary slots and emission of the dumped
+    // rows write disjoint rows, side by side on the two tail streams, after
+    // everything above on the controls' stream
+    CLB_HIP(hipEventCreateWithFlags(&p.joined, hipEventDisableTiming), EVENT_WAITING_ERROR);
+    CLB_HIP(hipEventCreateWithFlags(&p.done, hipEventDisableTiming), EVENT_WAITING_ERROR);
+    CLB_HIP(hipEventRecord(ctl->ev_fork, ctl->stream), EVENT_WAITING_ERROR);
+    p.tail = true;
+    hipStream_t t0s = ctl->tail[0], t1s = ctl->tail[1];
+    CLB_HIP(hipStreamWaitEvent(t0s, ctl->ev_fork, 0), EVENT_WAITING_ERROR);
+    CLB_HIP(hipStreamWaitEvent(t1s, ctl->ev_fork, 0), EVENT_WAITING_ERROR);
+    p.lt.begin(t0s);
+    compact_rows(t0s, nrows, nnz, p.nnz_row.ptr, p.offs.ptr, p.temp.ptr, p.toff.ptr, p.side.ptr, p.soff.ptr,
+                 nfail ? p.key2.ptr : nullptr, p.dump.on ? p.dump.isdense.ptr : nullptr, pred, u32p(cols),
+                 p.err.ptr, ctl->tuning.reserved[CLB_TUNE_COMPACT_TILES] != 0);
+    p.lt.end((int)CLB_KF_COMPACT, t0s);
+    emit_launch(t1s, ndense, p.perm.ptr, p.h, p.dump, p.gpre.ptr, p.offs.ptr, u32p(cols), &p.lt, CLB_KP_ONEPASS);
+    CLB_HIP(hipEventRecord(p.joined, t0s), EVENT_WAITING_ERROR);
+    CLB_HIP(hipStreamWaitEvent(t1s, p.joined, 0), EVENT_WAITING_ERROR);
+    CLB_HIP(hipEventRecord(p.done, t1s), EVENT_WAITING_ERROR);
+    return P;
+}
+
+bool one_pass_replan(const OnePass &p) { return p.replan; }
+
+u64 one_pass_bytes(const OnePass &p) {
+    u64 b = p.temp.cap + p.dump.store.cap + p.dump.cnt.cap + p.side.cap + p.toff.cap + p.offs.cap +
+            p.soff.cap + p.nnz_row.cap + p.perm.cap + p.pos.cap;
+    if (p.out.cols) b += p.out.cols->cap;
+    return b;
+}
+
+void one_pass_finish(OnePass &p, clb_dcsr &out, clb_spgemm_stats &st) {
+    clb_controls_s *ctl = p.ctl;
+    if (p.done) CLB_HIP(hipEventSynchronize(p.done), EVENT_WAITING_ERROR);
+    p.tail = false;
+    if (p.err.ptr && p.st.nzr) {
+        double e = 0;
+        readback(ctl, &e, p.err.ptr, 8);
+        p.st.pred_abs_log2_err = e / p.st.nzr;
+    }
+    p.lt.collect(p.st);
+    if (p.timing)
+        p.st.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - p.t0).count();
+    out = p.out;
+    p.out = clb_dcsr{};  // handed over
+    st = p.st;
+}
+
+void spgemm_learned(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &b, u32 rb, u32 re, bool timing,
+                    clb_dcsr &out, clb_spgemm_stats &st, bool bound_only, const PlanRows *pr) {
+    OnePassPtr p = one_pass_start(ctl, a, b, rb, re, timing, bound_only, pr);
+    clb_dcsr r{};
+    one_pass_finish(*p, r, st);
+    out = r;
 }
 
 // ---------------------------------------------------------------------------

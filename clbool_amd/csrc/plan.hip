@@ -111,10 +111,40 @@ static void plan_create(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &
     p.info.ms_plan = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
-static void plan_run(clb_controls_s *ctl, clb_spgemm_plan_s &p, u32 k, bool timing, clb_dcsr &out,
-                     clb_spgemm_stats &st) {
+// Block k exceeded the bound: size[cuts[k], cuts[k+1]) holds its exact row
+// sizes now; cut those rows again.
+static void recut(clb_controls_s *ctl, clb_spgemm_plan_s &p, u32 k) {
+    const u32 rb = p.cuts[k], re = p.cuts[k + 1];
+    std::vector<u32> sz(re - rb);
+    readback(ctl, sz.data(), p.size.ptr + rb, 4ull * (re - rb));
+    std::vector<u64> pre(re - rb + 1, 0);
+    for (u32 i = 0; i < re - rb; ++i) pre[i + 1] = pre[i] + sz[i];
+    std::vector<u32> sub;
+    cut_rows(pre.data(), 0, re - rb, p.bound, sub);
+    std::vector<u32> ends;
+    for (u32 e : sub) ends.push_back(rb + e);
+    ends.pop_back();  // == re, already cuts[k + 1]
+    p.cuts.insert(p.cuts.begin() + k + 1, ends.begin(), ends.end());
+    p.info.replans += 1;
+    p.info.nblocks = static_cast<u32>(p.cuts.size() - 1);
+}
+
+static PlanRows rows_of(clb_spgemm_plan_s &p, u32 k, bool *replan) {
+    const u32 rb = p.cuts[k], re = p.cuts[k + 1];
+    return PlanRows{&p.s, p.s.P + rb, p.size.ptr + rb, p.pred.ptr ? p.pred.ptr + rb : nullptr,
+                    p.ppre[re] - p.ppre[rb],
+                    // a single row is never cut further
+                    re - rb > 1 ? p.bound : 0xFFFFFFFFull, replan};
+}
+
+static void check_run(clb_controls_s *ctl, const clb_spgemm_plan_s &p, u32 k) {
     if (ctl != p.ctl) raise(INVALID_ARGUMENT, "a plan runs on the controls it was created with");
     if (k + 1 >= p.cuts.size()) raise(INVALID_ARGUMENT, "block index outside the plan");
+}
+
+static void plan_run(clb_controls_s *ctl, clb_spgemm_plan_s &p, u32 k, bool timing, clb_dcsr &out,
+                     clb_spgemm_stats &st) {
+    check_run(ctl, p, k);
     std::memset(&st, 0, sizeof(st));
     if (p.empty) {
         out = clb_dcsr{};
@@ -127,10 +157,7 @@ static void plan_run(clb_controls_s *ctl, clb_spgemm_plan_s &p, u32 k, bool timi
     for (;;) {
         const u32 rb = p.cuts[k], re = p.cuts[k + 1];
         bool replan = false;
-        PlanRows pr{&p.s, p.s.P + rb, p.size.ptr + rb, p.pred.ptr ? p.pred.ptr + rb : nullptr,
-                    p.ppre[re] - p.ppre[rb],
-                    // a single row is never cut further
-                    re - rb > 1 ? p.bound : 0xFFFFFFFFull, &replan};
+        PlanRows pr = rows_of(p, k, &replan);
         clb_dcsr r{};
         std::memset(&st, 0, sizeof(st));
         if (p.mode == CLB_SPGEMM_PLAIN)
@@ -141,23 +168,71 @@ static void plan_run(clb_controls_s *ctl, clb_spgemm_plan_s &p, u32 k, bool timi
             out = r;
             break;
         }
-        // size[rb, re) now holds the block's exact row sizes: cut it again
-        std::vector<u32> sz(re - rb);
-        readback(ctl, sz.data(), p.size.ptr + rb, 4ull * (re - rb));
-        std::vector<u64> pre(re - rb + 1, 0);
-        for (u32 i = 0; i < re - rb; ++i) pre[i + 1] = pre[i] + sz[i];
-        std::vector<u32> sub;
-        cut_rows(pre.data(), 0, re - rb, p.bound, sub);
-        std::vector<u32> ends;
-        for (u32 e : sub) ends.push_back(rb + e);
-        ends.pop_back();  // == re, already cuts[k + 1]
-        p.cuts.insert(p.cuts.begin() + k + 1, ends.begin(), ends.end());
-        p.info.replans += 1;
-        p.info.nblocks = static_cast<u32>(p.cuts.size() - 1);
+        recut(ctl, p, k);
     }
     if (!timing)
         st.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     ctl->last = st;
+}
+
+// Every block in order, each handed to fn as soon as it is complete.  In
+// the one-pass modes block k + 1's passes start while block k's tail
+// (compaction and dense-row emission) still runs on the tail streams.
+static void plan_run_all(clb_controls_s *ctl, clb_spgemm_plan_s &p, bool timing, clb_spgemm_block_fn fn,
+                         void *user) {
+    if (ctl != p.ctl) raise(INVALID_ARGUMENT, "a plan runs on the controls it was created with");
+    bool go = true;
+    auto deliver = [&](u32 k, clb_dcsr &blk, clb_spgemm_stats &st) {
+        ctl->last = st;
+        if (!go) {  // stopped: results started before the stop are dropped
+            clb_dcsr_release(&blk);
+            return;
+        }
+        go = fn(user, k, &blk, &st) == 0;  // fn owns blk now
+    };
+    if (p.empty || p.mode == CLB_SPGEMM_PLAIN) {
+        for (u32 k = 0; go && k + 1 < p.cuts.size(); ++k) {
+            clb_dcsr r{};
+            clb_spgemm_stats st;
+            plan_run(ctl, p, k, timing, r, st);
+            deliver(k, r, st);
+        }
+        return;
+    }
+    OnePassPtr pend;
+    u32 pend_k = 0;
+    auto finish_pending = [&]() {
+        clb_dcsr r{};
+        clb_spgemm_stats st;
+        one_pass_finish(*pend, r, st);
+        pend.reset();
+        deliver(pend_k, r, st);
+    };
+    for (u32 k = 0; go && k + 1 < p.cuts.size(); ++k) {
+        if (pend) {
+            // overlap only when there is room for a second block of about
+            // the pending one's size beside it (with room to spare for its
+            // dense-row dump); otherwise let the pending block finish first
+            // and hand its memory back to the pool
+            size_t free_b = 0, total_b = 0;
+            if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) (void)hipGetLastError();
+            const u64 avail = free_b + pool_for(ctl->device).cached_bytes();
+            if (avail < 3 * one_pass_bytes(*pend)) finish_pending();
+        }
+        OnePassPtr cur;
+        for (;;) {
+            bool replan = false;
+            PlanRows pr = rows_of(p, k, &replan);
+            cur = one_pass_start(ctl, p.a, p.b, p.cuts[k], p.cuts[k + 1], timing, p.mode == CLB_SPGEMM_BOUND, &pr);
+            if (!replan) break;
+            cur.reset();
+            recut(ctl, p, k);
+        }
+        if (pend) finish_pending();
+        pend = std::move(cur);
+        pend_k = k;
+    }
+    if (pend) finish_pending();
 }
 
 }  // namespace spg
@@ -207,6 +282,14 @@ clb_status clb_spgemm_plan_run(clb_controls ctl, clb_spgemm_plan plan, uint32_t 
         spg::plan_run(ctl, *plan, k, timing != 0, r, st);
         *out = r;
         if (stats) *stats = st;
+    });
+}
+
+clb_status clb_spgemm_plan_run_all(clb_controls ctl, clb_spgemm_plan plan, uint32_t timing, clb_spgemm_block_fn fn,
+                                   void *user) {
+    return run_guarded([&] {
+        if (!plan || !fn) raise(INVALID_ARGUMENT, "null argument");
+        spg::plan_run_all(ctl, *plan, timing != 0, fn, user);
     });
 }
 

@@ -150,8 +150,8 @@ __global__ __launch_bounds__(EMIT_WG) void k_bitmap_emit(const uint4 *__restrict
                                                          u32 words_per_row, u32 *__restrict__ cols) {
     constexpr u32 NW = EMIT_WG / 64, BATCH = 16, SPAN = BATCH * 64;
     __shared__ u32 stage[NW][dev::STAGE_WORDS];
-    const u32 slot = list[blockIdx.x];
-    const u32 r = perm[slot];
+    const u32 slot = blockIdx.x;  // slots are claimed densely from 0
+    const u32 r = perm[list[slot]];
     u32 *out = cols + off[r];
     const u64 lim = off[r + 1] - off[r];
     const u32 wv = dev::wave_id(), l = (u32)dev::lane();
@@ -181,7 +181,18 @@ __global__ __launch_bounds__(EMIT_WG) void k_bitmap_emit(const uint4 *__restrict
 }
 
 // ---------------------------------------------------------------------------
-void setup_dump(clb_controls_s *ctl, const Setup &s, const BinHost &h, DumpBufs &d, Out &o, bool one_pass) {
+// Rows of the bitmap range of perm whose size key (capacity or exact size)
+// reaches thr: the rows likely to be dumped.
+__global__ void k_dump_candidates(const u32 *__restrict__ perm, u32 n, const u32 *__restrict__ key, u32 thr,
+                                  u32 *__restrict__ count) {
+    const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool hit = i < n && key[perm[i]] >= thr;
+    const unsigned long long m = __ballot(hit);
+    if (dev::lane() == 0 && m) atomicAdd(count, (u32)__popcll(m));
+}
+
+void setup_dump(clb_controls_s *ctl, const Setup &s, const BinHost &h, DumpBufs &d, Out &o, bool one_pass,
+                const u32 *perm, const u32 *key) {
     o.dump = Dump{};
     d.on = false;
     // default: on in the one pass (its in-kernel emission writes a temporary
@@ -195,7 +206,23 @@ void setup_dump(clb_controls_s *ctl, const Setup &s, const BinHost &h, DumpBufs 
     if (d.nbm == 0 || (nslices << s.cfg.bm_bits) > (1ull << 26)) return;
     d.words = (u32)(nslices << s.cfg.bm_bits);
     d.nch = d.words / EMIT_CHUNK;
-    const u64 bytes = (u64)d.nbm * d.words * 4;
+    const u32 dense_min = ((1u << s.cfg.bm_bits) / 1024u) * knob;
+    // slots for the rows likely to be dumped: those whose size key reaches a
+    // quarter of the dense threshold over all slices, plus a margin for rows
+    // the one pass dumps because they outgrow their capacity
+    u32 slots = d.nbm;
+    if (key && perm) {
+        DevArray<u32> cnt(ctl, 1);
+        fill_u32(ctl, cnt.ptr, 1, 0u);
+        const u64 thr = (u64)dense_min * nslices / 4;
+        k_dump_candidates<<<div_up(d.nbm, 256), 256, 0, ctl->stream>>>(
+            perm + h.off[BIN_BITMAP0], d.nbm, key, (u32)(thr < 0xFFFFFFFFull ? thr : 0xFFFFFFFFull), cnt.ptr);
+        CLB_LAUNCHED("dump_candidates");
+        u32 cand = 0;
+        readback(ctl, &cand, cnt.ptr, 4);
+        const u64 want = (u64)cand + cand / 8 + 256;
+        slots = want < d.nbm ? (u32)want : d.nbm;
+    }
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
         (void)hipGetLastError();
@@ -203,41 +230,57 @@ void setup_dump(clb_controls_s *ctl, const Setup &s, const BinHost &h, DumpBufs 
     }
     // keep most of HBM for the operands and the output; blocks cached by the
     // pool count as free (the pool hands them back when an allocation fails)
-    if (bytes > (free_b + pool_for(ctl->device).cached_bytes()) / 3) return;
-    d.store.reset(ctl, (u64)d.nbm * d.words / 4);
-    d.cnt.reset(ctl, (u64)d.nbm * d.nch);
-    d.list.reset(ctl, d.nbm);
+    const u64 budget = (free_b + pool_for(ctl->device).cached_bytes()) / 3;
+    const u64 per_slot = (u64)d.words * 4;
+    if ((u64)slots * per_slot > budget) slots = (u32)(budget / per_slot);
+    if (slots == 0) return;
+    d.slots = slots;
+    d.store.reset(ctl, (u64)slots * d.words / 4);
+    d.cnt.reset(ctl, (u64)slots * d.nch);
+    d.list.reset(ctl, slots);
     d.count.reset(ctl, 1);
     d.isdense.reset(ctl, s.nrows);
     fill_u32(ctl, d.cnt.ptr, d.cnt.n, 0);
     fill_u32(ctl, d.count.ptr, 1, 0);
     fill_u32(ctl, d.isdense.ptr, s.nrows, 0);
     // the decision is taken on slice 0's count
-    o.dump = Dump{d.store.ptr, d.cnt.ptr, d.list.ptr, d.count.ptr, d.isdense.ptr,
-                  ((1u << s.cfg.bm_bits) / 1024u) * knob};
+    o.dump = Dump{d.store.ptr, d.cnt.ptr, d.list.ptr, d.count.ptr, d.isdense.ptr, dense_min, slots};
     d.on = true;
 }
 
-void emit_dense(clb_controls_s *ctl, const u32 *perm, const BinHost &h, DumpBufs &d, const u64 *off, u32 *cols,
-                LaunchTimer *lt, int pass, clb_spgemm_stats *st) {
-    if (!d.on) return;
+u32 emit_prepare(clb_controls_s *ctl, DumpBufs &d, DevArray<u64> &gpre, clb_spgemm_stats *st) {
+    if (!d.on) return 0;
     u32 count = 0;
     readback(ctl, &count, d.count.ptr, 4);
-    if (!count) return;
-    DevArray<u64> gpre(ctl, d.cnt.n + 1);
-    exclusive_scan_u32_to_u64(ctl, d.cnt.ptr, gpre.ptr, d.cnt.n, gpre.ptr + d.cnt.n);
+    if (count > d.slots) count = d.slots;  // claims beyond the last slot were refused
+    if (!count) return 0;
     if (d.words % (1u << 14)) raise(RUN_KERNEL_FAILURE, "dumped rows must hold whole 2^14-word windows");
-    if (lt) lt->begin(ctl->stream);
-    k_bitmap_emit<<<count, EMIT_WG, 0, ctl->stream>>>(d.store.ptr, d.nch, gpre.ptr, d.list.ptr,
-                                                       perm + h.off[BIN_BITMAP0], off, d.words, cols);
-    CLB_LAUNCHED("bitmap_emit");
-    if (lt) lt->end((int)CLB_KF(pass, CLB_KK_EMIT), ctl->stream);
+    gpre.reset(ctl, d.cnt.n + 1);
+    exclusive_scan_u32_to_u64(ctl, d.cnt.ptr, gpre.ptr, d.cnt.n, gpre.ptr + d.cnt.n);
     if (st) {
         u64 total = 0;
         readback(ctl, &total, gpre.ptr + d.cnt.n, 8);
         st->dense_rows += count;
         st->dense_nnz += total;
     }
+    return count;
+}
+
+void emit_launch(hipStream_t es, u32 count, const u32 *perm, const BinHost &h, const DumpBufs &d, const u64 *gpre,
+                 const u64 *off, u32 *cols, LaunchTimer *lt, int pass) {
+    if (!count) return;
+    if (lt) lt->begin(es);
+    k_bitmap_emit<<<count, EMIT_WG, 0, es>>>(d.store.ptr, d.nch, gpre, d.list.ptr, perm + h.off[BIN_BITMAP0], off,
+                                            d.words, cols);
+    CLB_LAUNCHED("bitmap_emit");
+    if (lt) lt->end((int)CLB_KF(pass, CLB_KK_EMIT), es);
+}
+
+void emit_dense(clb_controls_s *ctl, const u32 *perm, const BinHost &h, DumpBufs &d, const u64 *off, u32 *cols,
+                LaunchTimer *lt, int pass, clb_spgemm_stats *st) {
+    DevArray<u64> gpre;  // released after the launch: later work on this stream is ordered after it
+    const u32 count = emit_prepare(ctl, d, gpre, st);
+    emit_launch(ctl->stream, count, perm, h, d, gpre.ptr, off, cols, lt, pass);
 }
 
 BinCfg make_bincfg(const clb_controls_s *ctl, u32 ncols) {
@@ -514,15 +557,26 @@ void launch_bins(clb_controls_s *ctl, const Setup &s, const u32 *perm, const u32
         const u32 *pp = perm + off[b];
         hipStream_t st = next();
         t.begin(st);
-        switch (bits) {
-            case 7: DISPATCH3(k_hash, n, 64, st, 7, 64); break;
-            case 8: DISPATCH3(k_hash, n, 64, st, 8, 64); break;
-            case 9: DISPATCH3(k_hash, n, 64, st, 9, 64); break;
-            case 10: DISPATCH3(k_hash, n, 128, st, 10, 128); break;
-            case 11: DISPATCH3(k_hash, n, 128, st, 11, 128); break;
-            case 12: DISPATCH3(k_hash, n, 256, st, 12, 256); break;
-            case 13: DISPATCH3(k_hash, n, 512, st, 13, 512); break;
-            case 14: DISPATCH3(k_hash, n, 1024, st, 14, 1024); break;
+        // workgroup shape per table size: the default, or twice the threads
+        // when the table size's bit of the hash_wide knob is set
+        const bool wide = (ctl->tuning.reserved[CLB_TUNE_HASH_WIDE] >> (bits - HASH_MIN_BITS)) & 1u;
+        switch (bits * 2 + (wide ? 1 : 0)) {
+            case 14: DISPATCH3(k_hash, n, 64, st, 7, 64); break;
+            case 15: DISPATCH3(k_hash, n, 128, st, 7, 128); break;
+            case 16: DISPATCH3(k_hash, n, 64, st, 8, 64); break;
+            case 17: DISPATCH3(k_hash, n, 128, st, 8, 128); break;
+            case 18: DISPATCH3(k_hash, n, 64, st, 9, 64); break;
+            case 19: DISPATCH3(k_hash, n, 128, st, 9, 128); break;
+            case 20: DISPATCH3(k_hash, n, 128, st, 10, 128); break;
+            case 21: DISPATCH3(k_hash, n, 256, st, 10, 256); break;
+            case 22: DISPATCH3(k_hash, n, 128, st, 11, 128); break;
+            case 23: DISPATCH3(k_hash, n, 256, st, 11, 256); break;
+            case 24: DISPATCH3(k_hash, n, 256, st, 12, 256); break;
+            case 25: DISPATCH3(k_hash, n, 512, st, 12, 512); break;
+            case 26: DISPATCH3(k_hash, n, 512, st, 13, 512); break;
+            case 27: DISPATCH3(k_hash, n, 1024, st, 13, 1024); break;
+            case 28:
+            case 29: DISPATCH3(k_hash, n, 1024, st, 14, 1024); break;
         }
         CLB_LAUNCHED("hash");
         t.end(fam(1), st);
@@ -706,7 +760,7 @@ void spgemm_plain(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &b, u32
     try {
         Out o2{nnz_row, offs.ptr, u32p(cols), nullptr, reinterpret_cast<u32 *>(s.scal.ptr + 3), Dump{}, 0};
         DumpBufs dump;
-        setup_dump(ctl, s, h2, dump, o2, false);
+        setup_dump(ctl, s, h2, dump, o2, false, perm2.ptr, nnz_row);
         ensure_splits(ctl, s, h2);
         launch_bins(ctl, s, perm2.ptr, h2.off, NUM, o2, &lt, -1);
         emit_dense(ctl, perm2.ptr, h2, dump, offs.ptr, u32p(cols), &lt, CLB_KP_NUMERIC, timing ? &st : nullptr);

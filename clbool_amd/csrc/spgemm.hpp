@@ -68,17 +68,25 @@ struct BinHost {
 // Scratch of the dense-bitmap path (see Dump in spgemm_kernels.hpp).
 struct DumpBufs {
     bool on = false;
-    u32 nbm = 0, nch = 0, words = 0;
+    u32 nbm = 0, nch = 0, words = 0, slots = 0;
     DevArray<uint4> store;
     DevArray<u32> cnt, list, count, isdense;
 };
 // Allocates the dense-bitmap scratch for the bitmap bin of `h` (nothing when
 // the window is below 2^14 words, the knob disables it or it would not fit)
-// and fills o.dump.
-void setup_dump(clb_controls_s *ctl, const Setup &s, const BinHost &h, DumpBufs &d, Out &o, bool one_pass);
+// and fills o.dump.  With perm and key (the rows' capacities or exact
+// sizes), slots only for the rows likely to be dumped; else one per bitmap row.
+void setup_dump(clb_controls_s *ctl, const Setup &s, const BinHost &h, DumpBufs &d, Out &o, bool one_pass,
+                const u32 *perm = nullptr, const u32 *key = nullptr);
 // Expands the dumped rows into cols at the exact row offsets `off`.
 void emit_dense(clb_controls_s *ctl, const u32 *perm, const BinHost &h, DumpBufs &d, const u64 *off, u32 *cols,
                 LaunchTimer *lt, int pass, clb_spgemm_stats *st);
+// The same in two steps: the count and the chunk prefix into gpre on the
+// controls' stream (returns the number of dumped rows), then the emission
+// kernel on stream `es` (which must be ordered after the prefix).
+u32 emit_prepare(clb_controls_s *ctl, DumpBufs &d, DevArray<u64> &gpre, clb_spgemm_stats *st);
+void emit_launch(hipStream_t es, u32 count, const u32 *perm, const BinHost &h, const DumpBufs &d, const u64 *gpre,
+                 const u64 *off, u32 *cols, LaunchTimer *lt, int pass);
 
 BinCfg make_bincfg(const clb_controls_s *ctl, u32 ncols);
 void ensure_splits(clb_controls_s *ctl, Setup &s, const BinHost &h);
@@ -117,6 +125,20 @@ void plan_sizes_plain(clb_controls_s *ctl, Setup &s, u32 *size);
 void spgemm_learned(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &b, u32 row_begin, u32 row_end,
                     bool timing, clb_dcsr &out, clb_spgemm_stats &st, bool bound_only = false,
                     const PlanRows *pr = nullptr);
+// The one pass of one block in two halves: start runs everything up to the
+// tail (compaction and dense-row emission), which it leaves running on the
+// controls' tail streams; finish waits for it and hands over the result.  A
+// block that exceeded a plan's bound (one_pass_replan) has no result.
+struct OnePass;
+struct OnePassDeleter {
+    void operator()(OnePass *p) const;  // waits for the block's tail first
+};
+using OnePassPtr = std::unique_ptr<OnePass, OnePassDeleter>;
+OnePassPtr one_pass_start(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &b, u32 row_begin, u32 row_end,
+                          bool timing, bool bound_only, const PlanRows *pr);
+bool one_pass_replan(const OnePass &p);
+u64 one_pass_bytes(const OnePass &p);  // device memory the block holds
+void one_pass_finish(OnePass &p, clb_dcsr &out, clb_spgemm_stats &st);
 // Planning pass of the one pass: s prepared over all rows (products in s.P
 // afterwards), capacities into size, predictions into pred (learned only).
 void plan_sizes_learned(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &b, Setup &s, u32 *size,

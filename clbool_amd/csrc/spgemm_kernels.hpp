@@ -447,13 +447,16 @@ constexpr u32 FAILED = 0xFFFFFFFFu;
 // chunk; k_bitmap_emit expands them later into their final positions with
 // coalesced stores.  store == nullptr disables this path.
 constexpr u32 EMIT_CHUNK = 1024;  // words per emission chunk
+// Slots are handed out in the order rows decide to dump (*count counts the
+// claims); a row that finds all `capacity` slots taken is not dumped.
 struct Dump {
-    uint4 *store;      // [slot][W/4]: slot = the row's index in the bitmap range of perm
-    u32 *chunk_cnt;    // [slot][W / EMIT_CHUNK] popcounts (zero for rows not dumped)
-    u32 *list;         // slots of the dumped rows
-    u32 *count;        // number of dumped rows
+    uint4 *store;      // [slot][nslices][W/4]
+    u32 *chunk_cnt;    // [slot][nslices][W / EMIT_CHUNK] popcounts (zero for unused slots)
+    u32 *list;         // [slot]: the dumped row's index in the bitmap range of perm
+    u32 *count;        // slots claimed (may exceed capacity)
     u32 *isdense;      // [row] 1 if the row was dumped
-    u32 dense_min;     // dump rows with at least this many columns
+    u32 dense_min;     // dump rows with at least this many columns in their first window
+    u32 capacity;      // slots in store
 };
 
 struct Out {
@@ -715,6 +718,7 @@ __device__ __forceinline__ void bitmap_row(const Mats &g, const u32 *__restrict_
     u32 mine = 0;
     u64 written = 0;
     bool dumping = false;  // uniform: the row goes to the dense-row emission
+    u32 slot = 0;          // its slot in the dump
     u32 *out = MODE == SYM ? nullptr : o.cols + o.off[r];
     const u64 lim = row_limit<MODE>(o, r);
     for (u32 s = 0; s < nslices; ++s) {
@@ -765,9 +769,18 @@ __device__ __forceinline__ void bitmap_row(const Mats &g, const u32 *__restrict_
             const u32 n = dev::block_sum<u32, WG>(cnt, red);
             // a one-pass row over its capacity already in its first window
             // is dumped too: a dumped row needs no temporary slot
-            if (s == 0) dumping = n >= o.dump.dense_min || (MODE == ONE && n > lim);
+            if (s == 0) {
+                dumping = n >= o.dump.dense_min || (MODE == ONE && n > lim);  // n is uniform
+                if (dumping) {
+                    // claim a slot (big is free between gathers)
+                    if (threadIdx.x == 0) big[0] = atomicAdd(o.dump.count, 1u);
+                    __syncthreads();
+                    slot = big[0];
+                    __syncthreads();
+                    dumping = slot < o.dump.capacity;
+                }
+            }
             if (dumping) {
-                const u32 slot = idx;
                 uint4 *dst = o.dump.store + ((u64)slot * nslices + s) * (W / 4);
                 for (u32 q = threadIdx.x; q < W / 4; q += WG) dst[q] = bm4[q];
                 if (threadIdx.x % TPC == 0)
@@ -829,9 +842,11 @@ __device__ __forceinline__ void bitmap_row(const Mats &g, const u32 *__restrict_
         // exact offset, its capacity does not matter); in the numeric pass
         // when it matches the symbolic count
         if (threadIdx.x == 0) {
+            // every claimed slot is emitted (the emission never writes past
+            // the row's exact size)
+            o.dump.list[slot] = idx;
             const bool fits = MODE == ONE ? written <= 0xFFFFFFFFull : written == lim;
             if (fits) {
-                o.dump.list[atomicAdd(o.dump.count, 1u)] = idx;
                 o.dump.isdense[r] = 1;
                 if (MODE == ONE) o.nnz_row[r] = (u32)written;
             } else if (MODE == ONE) {

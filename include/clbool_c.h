@@ -99,12 +99,19 @@ typedef struct clb_tuning {
                                       reserved[CLB_TUNE_COMPACT_TILES]: nonzero:
                                       the one pass's compaction gives every
                                       workgroup an equal tile of output columns
-                                      (0: one workgroup per row)                */
+                                      (0: one workgroup per row)
+                                      reserved[CLB_TUNE_HASH_WIDE]: bit b set:
+                                      LDS hash tables of 2^(7+b) slots run with
+                                      twice the default workgroup (64 threads
+                                      for 2^7..2^9 slots, 128 for 2^10..2^11,
+                                      256 for 2^12, 512 for 2^13; 2^14: 1024
+                                      either way)                               */
 } clb_tuning;
 #define CLB_TUNE_BITMAP_PRED_CAPS 0
 #define CLB_TUNE_GATHER_COMBINE   1
 #define CLB_TUNE_ESC_MAX_PRODUCTS 2
 #define CLB_TUNE_COMPACT_TILES    3
+#define CLB_TUNE_HASH_WIDE        4
 
 #define CLB_SPGEMM_PLAIN   0u  /* two-pass: symbolic count, scan, numeric      */
 #define CLB_SPGEMM_LEARNED 1u  /* one pass sized by the learned predictor,
@@ -284,6 +291,15 @@ clb_status clb_spgemm_plan_info_get(clb_spgemm_plan plan, clb_spgemm_plan_info *
 clb_status clb_spgemm_plan_block(clb_spgemm_plan plan, uint32_t k, uint32_t *row_begin, uint32_t *row_end);
 clb_status clb_spgemm_plan_run(clb_controls ctl, clb_spgemm_plan plan, uint32_t k, uint32_t timing, clb_dcsr *out,
                                clb_spgemm_stats *stats);
+/* Runs every block in order and hands each one to fn as soon as it is
+ * complete: fn(user, k, &block, &stats) owns the block (it releases it with
+ * clb_dcsr_release) and returns 0 to go on, nonzero to stop.  In the one-pass
+ * modes block k + 1's passes start while block k's last phase (compaction and
+ * dense-row emission) still runs, so the blocks overlap; fn is called on the
+ * calling thread.  Results are the same as clb_spgemm_plan_run's. */
+typedef int32_t (*clb_spgemm_block_fn)(void *user, uint32_t k, clb_dcsr *block, const clb_spgemm_stats *stats);
+clb_status clb_spgemm_plan_run_all(clb_controls ctl, clb_spgemm_plan plan, uint32_t timing, clb_spgemm_block_fn fn,
+                                   void *user);
 clb_status clb_spgemm_plan_release(clb_spgemm_plan plan);
 
 /* dcsr::kronecker_product (src/dcsr/dcsr_kronecker_product.cpp:6-65) */

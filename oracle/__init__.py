@@ -42,7 +42,9 @@ def lib():
     """Load liboracle.so (building it with make when it is missing)."""
     global _LIB
     if _LIB is None:
-        path = os.path.join(_HERE, "liboracle.so")
+        # CLBOOL_ORACLE_LIB: another build of the same source (the sanitizer
+        # build of the Makefile's `sanitize` target)
+        path = os.environ.get("CLBOOL_ORACLE_LIB") or os.path.join(_HERE, "liboracle.so")
         if not os.path.exists(path):
             build()
         L = C.CDLL(path)

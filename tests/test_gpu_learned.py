@@ -217,7 +217,7 @@ def test_trained_network_gains_over_sizing_without_it(controls, trained):
     t, z, bnd = res["trained"], res["zero"], res["bound"]
     assert t["fallback"] < 0.05 * A.nzr and t["fallback"] * 5 < z["fallback"]
     assert bnd["fallback"] == 0
-    # bitmap rows always reserve their bound; hash rows reserve the prediction
-    assert ref.nnz <= t["capacity"] < bnd["capacity"]
+    # every row reserves its prediction (with the margin), far below the bound
+    assert 0.8 * ref.nnz <= t["capacity"] < bnd["capacity"]
     assert t["err"] < 0.5
     assert t["ms"] < z["ms"]

@@ -122,3 +122,56 @@ def test_bound_mode_and_bitmap_caps_exact(controls, bitmap_pred, combine):
                     assert st["capacity"] == int(np.minimum(oracle.workload(A, B), B.ncols).sum())
     finally:
         at.apply(controls, base)
+
+
+@pytest.mark.parametrize("mode", [cb.SPGEMM_PLAIN, cb.SPGEMM_LEARNED, cb.SPGEMM_BOUND])
+def test_run_all_matches_block_runs(controls, mode):
+    """Pipelined runs (the next block's passes overlap a one-pass block's
+    compaction and emission) give every block exactly, in order; stopping
+    early drops the rest."""
+    n = 1 << 14
+    A, a = pairs_dcsr(controls, *gen.rmat(14, 16, 43), n, n)
+    full = oracle.spgemm(A, A)
+    cb.predictor_set_params(controls, _over_model())
+    bound = max(full.nnz // 6, n)
+    seen = []
+
+    def take(k, C, st):
+        seen.append((k, C, st))
+
+    plan = cb.dcsr.plan(controls, a, a, mode=mode, block_bound=bound)
+    plan.run_all(take)
+    assert [k for k, _, _ in seen] == list(range(plan.nblocks)) and len(seen) >= 6
+    for k, C, st in seen:
+        lo, hi = plan.block(k)
+        ref = oracle.spgemm(A, A, row_sel=np.arange(lo, hi, dtype=np.uint32))
+        assert_dcsr_equal(controls, C, ref, f"run_all mode {mode} block {k}")
+    # the same blocks from single runs
+    for k, C, st in seen[:2]:
+        D, _ = plan.run(k)
+        assert cb.dcsr.digest(controls, D) == cb.dcsr.digest(controls, C)
+    # a callback that stops after two blocks
+    got = []
+    cb.dcsr.plan(controls, a, a, mode=mode, block_bound=bound).run_all(lambda k, C, st: got.append(k) or k == 1)
+    assert got == [0, 1]
+
+
+def test_run_all_recuts_under_pipelining(controls):
+    cb.predictor_set_params(controls, np.zeros(cb.MLP_PARAMS, np.float32))
+    n = 1 << 14
+    A, a = pairs_dcsr(controls, *gen.rmat(14, 16, 44), n, n)
+    full = oracle.spgemm(A, A)
+    bound = max(full.nnz // 5, n)
+    plan = cb.dcsr.plan(controls, a, a, mode=cb.SPGEMM_LEARNED, block_bound=bound)
+    blocks = []
+    plan.run_all(lambda k, C, st: blocks.append((k, C)))
+    assert plan.info()["replans"] > 0 and len(blocks) == plan.nblocks
+    rows = 0
+    for k, C in blocks:
+        lo, hi = plan.block(k)
+        assert lo == rows
+        ref = oracle.spgemm(A, A, row_sel=np.arange(lo, hi, dtype=np.uint32))
+        assert_dcsr_equal(controls, C, ref, f"recut pipelined block {k}")
+        assert C.nnz <= bound or hi - lo == 1
+        rows = hi
+    assert rows == A.nzr
