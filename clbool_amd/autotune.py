@@ -32,14 +32,17 @@ KNOBS = {
     "gather_combine": [0, 1],     # bitmap gather: one LDS atomic per word and wave (1) or per product (0)
     "esc_max_products": [0, 2048, 8192],  # multi-window bitmap rows up to this many products sort in LDS
     "compact_tiles": [0, 1],      # one pass: compaction by output tiles (1) or one workgroup per row (0)
+    "onepass_bitmap_min": [0, 8192, 32768, 131072],  # one pass: product count that forces a bitmap (0: never)
 }
 # workgroup shape of each LDS hash table size: default (0) or twice the threads (1)
 for _b in range(7, 14):
     KNOBS[f"hash_wide_{_b}"] = [0, 1]
 # knobs that live in clb_tuning.reserved[] (whole words, or one bit of a word)
-RESERVED = {"bitmap_pred_caps": 0, "gather_combine": 1, "esc_max_products": 2, "compact_tiles": 3}
+RESERVED = {"bitmap_pred_caps": 0, "gather_combine": 1, "esc_max_products": 2, "compact_tiles": 3,
+            "onepass_bitmap_min": 5}
 RESERVED_BITS = {f"hash_wide_{b}": (4, b - 7) for b in range(7, 14)}
-LEARNED_ONLY = ("learned_margin_pct", "bitmap_pred_caps", "compact_tiles")
+LEARNED_ONLY = ("learned_margin_pct", "bitmap_pred_caps")   # knobs of the predictor's sizing
+ONE_PASS_ONLY = ("compact_tiles", "onepass_bitmap_min")      # knobs of the learned and bound one pass
 
 
 def _get(t, k):
@@ -92,7 +95,14 @@ def autotune(controls, a, b, blocks, mode=0, repeats=3, knobs=None, log=None, pa
     knobs interact, so the sweep over all of them runs `passes` times)."""
     import clbool_amd as cb
 
-    knobs = knobs or [k for k in KNOBS if mode == cb.SPGEMM_LEARNED or k not in LEARNED_ONLY]
+    def relevant(k):
+        if k in LEARNED_ONLY:
+            return mode == cb.SPGEMM_LEARNED
+        if k in ONE_PASS_ONLY:
+            return mode in (cb.SPGEMM_LEARNED, cb.SPGEMM_BOUND)
+        return True
+
+    knobs = knobs or [k for k in KNOBS if relevant(k)]
 
     def run():
         best = None

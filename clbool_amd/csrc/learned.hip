@@ -431,7 +431,10 @@ static void prepare_features(clb_controls_s *ctl, const clb_dcsr &a, const clb_d
 
 static BinCfg learned_bincfg(const clb_controls_s *ctl, u32 ncols) {
     BinCfg c = make_bincfg(ctl, ncols);
-    c.bitmap_min = 0xFFFFFFFFu;  // the predicted size decides, not the product count
+    // the predicted size decides, and rows with at least onepass_bitmap_min
+    // products take a bitmap whatever their size (0: the size alone)
+    const u32 pmin = ctl->tuning.reserved[CLB_TUNE_ONEPASS_BITMAP_MIN];
+    c.bitmap_min = pmin ? pmin : 0xFFFFFFFFu;
     return c;
 }
 

@@ -105,13 +105,18 @@ typedef struct clb_tuning {
                                       twice the default workgroup (64 threads
                                       for 2^7..2^9 slots, 128 for 2^10..2^11,
                                       256 for 2^12, 512 for 2^13; 2^14: 1024
-                                      either way)                               */
+                                      either way)
+                                      reserved[CLB_TUNE_ONEPASS_BITMAP_MIN]: in
+                                      the one pass, rows with at least this many
+                                      products use a bitmap whatever their
+                                      predicted size (0: the size alone decides) */
 } clb_tuning;
 #define CLB_TUNE_BITMAP_PRED_CAPS 0
 #define CLB_TUNE_GATHER_COMBINE   1
 #define CLB_TUNE_ESC_MAX_PRODUCTS 2
 #define CLB_TUNE_COMPACT_TILES    3
 #define CLB_TUNE_HASH_WIDE        4
+#define CLB_TUNE_ONEPASS_BITMAP_MIN 5
 
 #define CLB_SPGEMM_PLAIN   0u  /* two-pass: symbolic count, scan, numeric      */
 #define CLB_SPGEMM_LEARNED 1u  /* one pass sized by the learned predictor,
