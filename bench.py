@@ -218,10 +218,15 @@ def bench_closure(cb, gen, ctl, max_iters, log, seed=0, legs=None, warm=True):
         ctl.sync()
         products, iters, steps_ms, step_nnz = 0, 0, [], []
         t0 = time.perf_counter()
+        step_mallocs = []
         for it in range(max_iters):
+            ps0 = ctl.pool_stats()
             ts = time.perf_counter()
             st = cb.closure_step(ctl, mats, triples, mode=mode, spgemm_mode=smode)
             steps_ms.append(1e3 * (time.perf_counter() - ts))
+            ps1 = ctl.pool_stats()
+            step_mallocs.append([ps1["mallocs"] - ps0["mallocs"], round((ps1["malloc_bytes"] - ps0["malloc_bytes"]) / 1e9, 2),
+                                 ps1["drops"] - ps0["drops"]])
             step_nnz.append(int(sum(m.nnz for m in mats)))
             products += st["products"]
             iters += 1
@@ -232,7 +237,8 @@ def bench_closure(cb, gen, ctl, max_iters, log, seed=0, legs=None, warm=True):
         finals[name] = [cb.dcsr.digest(ctl, m) for m in mats]
         out[name] = {"iterations": iters, "seconds": secs, "products": products,
                      "products_per_s": products / secs, "step_ms": [round(x, 2) for x in steps_ms],
-                     "step_nnz": step_nnz, "fixed_point": st["changed"] == 0, "final_nnz": step_nnz[-1]}
+                     "step_nnz": step_nnz, "fixed_point": st["changed"] == 0, "final_nnz": step_nnz[-1],
+                     "step_pool": step_mallocs}  # per step: hipMalloc calls, GB, cache drops
         log(f"[bench] closure {name}: {iters} steps, {secs * 1e3:.1f} ms, {products / secs:.3e} products/s")
     # full-matrix digests of all 64 labels
     out["identical_results"] = len({tuple(v) for v in finals.values()}) == 1
@@ -523,7 +529,12 @@ def bench_product(cb, gen, ctl, oracle, workload, modes, steps, warmup, tuned, v
             log(f"[bench] {workload} {m}: {validated[m]} sampled rows bit-exact vs oracle")
         ctl.trim()
         run_steps(cb, ctl, a_dev, MODE_CODES[m], warmup, False, block_bound)
+        ps0 = ctl.pool_stats()
         secs, agg = run_steps(cb, ctl, a_dev, MODE_CODES[m], steps, True, block_bound)
+        ps1 = ctl.pool_stats()
+        agg["pool_timed"] = {"mallocs": ps1["mallocs"] - ps0["mallocs"],
+                             "malloc_GB": (ps1["malloc_bytes"] - ps0["malloc_bytes"]) / 1e9,
+                             "cache_drops": ps1["drops"] - ps0["drops"]}
         ms = 1e3 * secs / steps
         results[m] = {"ms_per_step": ms, "products_per_s": agg["products"] / (ms / 1e3),
                       "nnz_per_s": agg["nnz"] / (ms / 1e3), "nnz": agg["nnz"], "agg": agg}
@@ -544,6 +555,10 @@ def bench_product(cb, gen, ctl, oracle, workload, modes, steps, warmup, tuned, v
             "pred_abs_log2_err": agg["err_sum"] / agg["err_rows"] if m == "learned" and agg["err_rows"] else None,
             "capacity_over_nnz": agg["capacity"] / max(1, agg["nnz"]) if m != "plain" else None,
             "dense_rows": agg["dense_rows"], "dense_nnz": agg["dense_nnz"], "tuning": tuned[m],
+            "pool_in_timed_steps": agg["pool_timed"],
+            # per bin (0 empty, 1 tiny, 2-9 hash 2^7..2^14 slots, 10-12 LDS ESC,
+            # 13-31 bitmap by product count), last timed step
+            "bin_rows": agg["bin_rows"], "bin_products": agg["bin_products"],
             "roofline": roofline_of(cb, agg, workload),
             "step_hbm_frac": step_algorithmic_bytes(int(rows.size), int(cols.size), total_products,
                                                     int(rows.size), agg["nnz"]) / (v["ms_per_step"] / 1e3) / 1e9
@@ -582,7 +597,8 @@ def main():
                     help="run only the hypersparse workload, training on this many instances")
     ap.add_argument("--closure-only", type=int, default=0,
                     help="run only the 64-label closure, up to this many steps or its fixed point, per-triple "
-                         "plain against batched learned, without the warm-up run")
+                         "plain against batched learned (each after a warm-up run of its own)")
+    ap.add_argument("--closure-cold", action="store_true", help="with --closure-only: no warm-up runs")
     ap.add_argument("--c5-only", action="store_true",
                     help="run only the automaton (x) R-MAT 22/32 workload: the Kronecker in automaton-row "
                          "blocks and a bounded sample of one K * K closure step")
@@ -616,7 +632,8 @@ def main():
         ctl = cb.create_controls()
         train_predictor(cb, gen, ctl, log)
         print(json.dumps({"closure": bench_closure(cb, gen, ctl, args.closure_only, log,
-                                                   legs=("per_triple_plain", "batched_learned"), warm=False)}),
+                                                   legs=("per_triple_plain", "batched_learned"),
+                                                   warm=not args.closure_cold)}),
               flush=True)
         return
     if args.hypersparse_only:

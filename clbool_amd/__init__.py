@@ -102,6 +102,13 @@ class Controls:
         _check(lib().clb_last_stats(self.handle, C.byref(s)))
         return s.as_dict()
 
+    def pool_stats(self) -> dict:
+        """Buffer-pool diagnostics: cached bytes, hipMalloc calls and bytes,
+        cache drops after a failed hipMalloc."""
+        a = (C.c_uint64 * 4)()
+        _check(lib().clb_pool_stats(self.handle, a))
+        return {"cached_bytes": a[0], "mallocs": a[1], "malloc_bytes": a[2], "drops": a[3]}
+
     def trim(self):
         _check(lib().clb_pool_trim(self.handle))
 

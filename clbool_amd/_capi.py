@@ -128,6 +128,7 @@ SIGNATURES = [
     ("clb_set_tuning", C.c_int32, [ctl_t, C.POINTER(Tuning)]),
     ("clb_last_stats", C.c_int32, [ctl_t, C.POINTER(SpgemmStats)]),
     ("clb_pool_trim", C.c_int32, [ctl_t]),
+    ("clb_pool_stats", C.c_int32, [ctl_t, C.POINTER(C.c_uint64)]),
     ("clb_buffer_create", C.c_int32, [ctl_t, C.c_uint64, C.POINTER(buf_t)]),
     ("clb_buffer_from_host", C.c_int32, [ctl_t, C.c_void_p, C.c_uint64, C.POINTER(buf_t)]),
     ("clb_buffer_read", C.c_int32, [ctl_t, buf_t, C.c_uint64, C.c_uint64, C.c_void_p]),

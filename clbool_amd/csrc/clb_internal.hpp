@@ -74,12 +74,22 @@ class Pool {
         return cached_;
     }
     int device() const { return device_; }
+    // [0] cached bytes, [1] hipMalloc calls, [2] bytes they allocated,
+    // [3] cache drops after a failed hipMalloc
+    void stats(u64 out[4]) const {
+        std::lock_guard<std::mutex> lk(mu_);
+        out[0] = cached_;
+        out[1] = mallocs_;
+        out[2] = malloc_bytes_;
+        out[3] = drops_;
+    }
 
   private:
     int device_;
     mutable std::mutex mu_;
     std::multimap<u64, void *> free_;  // capacity -> pointer
     u64 cached_ = 0;
+    u64 mallocs_ = 0, malloc_bytes_ = 0, drops_ = 0;
 };
 Pool &pool_for(int device);
 

@@ -58,6 +58,10 @@ void *Pool::alloc(u64 bytes, u64 *cap_out) {
     if (e != hipSuccess) {
         (void)hipGetLastError();
         trim();
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            ++drops_;
+        }
         e = hipMalloc(&p, cap);
     }
     if (prev != device_) (void)hipSetDevice(prev);
@@ -66,6 +70,11 @@ void *Pool::alloc(u64 bytes, u64 *cap_out) {
         std::ostringstream os;
         os << "device allocation of " << cap << " bytes failed: " << hipGetErrorString(e);
         raise(CREATE_BUFFER_ERROR, os.str());
+    }
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        ++mallocs_;
+        malloc_bytes_ += cap;
     }
     *cap_out = cap;
     return p;
@@ -409,6 +418,13 @@ clb_status clb_last_stats(clb_controls ctl, clb_spgemm_stats *out) {
 }
 
 clb_status clb_pool_trim(clb_controls ctl) { return guarded([&] { pool_for(ctl->device).trim(); }); }
+
+clb_status clb_pool_stats(clb_controls ctl, uint64_t out[4]) {
+    return guarded([&] {
+        if (!ctl || !out) raise(INVALID_ARGUMENT, "null argument");
+        pool_for(ctl->device).stats(out);
+    });
+}
 
 clb_status clb_buffer_create(clb_controls ctl, uint64_t bytes, clb_buffer *out) {
     return guarded([&] { *out = buffer_new(ctl, bytes); });

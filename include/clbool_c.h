@@ -203,6 +203,10 @@ clb_status clb_set_tuning(clb_controls ctl, const clb_tuning *t);
 clb_status clb_last_stats(clb_controls ctl, clb_spgemm_stats *out);
 /* Drop the device allocation cache of this controls' device. */
 clb_status clb_pool_trim(clb_controls ctl);
+/* Diagnostics of the device's buffer pool: out[0] bytes cached, out[1]
+ * hipMalloc calls so far, out[2] bytes they allocated, out[3] times the cache
+ * was dropped after a failed hipMalloc. */
+clb_status clb_pool_stats(clb_controls ctl, uint64_t out[4]);
 
 /* ---- buffers (replace cl::Buffer) --------------------------------------- */
 /* A zero-size buffer fails with CLBOOL_CREATE_BUFFER_ERROR, as cl::Buffer
