@@ -218,15 +218,19 @@ def bench_closure(cb, gen, ctl, max_iters, log, seed=0, legs=None, warm=True):
         ctl.sync()
         products, iters, steps_ms, step_nnz = 0, 0, [], []
         t0 = time.perf_counter()
-        step_mallocs = []
+        step_mallocs, step_phases = [], []
         for it in range(max_iters):
             ps0 = ctl.pool_stats()
             ts = time.perf_counter()
+            log(f"[bench] closure {name} step {it + 1} starts at monotonic {time.monotonic_ns()} ns")
             st = cb.closure_step(ctl, mats, triples, mode=mode, spgemm_mode=smode)
             steps_ms.append(1e3 * (time.perf_counter() - ts))
             ps1 = ctl.pool_stats()
             step_mallocs.append([ps1["mallocs"] - ps0["mallocs"], round((ps1["malloc_bytes"] - ps0["malloc_bytes"]) / 1e9, 2),
-                                 ps1["drops"] - ps0["drops"]])
+                                 ps1["drops"] - ps0["drops"], round(ps0["free_bytes"] / 1e9, 1),
+                                 round(ps0["cached_bytes"] / 1e9, 1)])
+            step_phases.append([round(st["ms_products"], 1), round(st["ms_unions"], 1), st["fallback_rows"],
+                                st["dense_rows"]])
             step_nnz.append(int(sum(m.nnz for m in mats)))
             products += st["products"]
             iters += 1
@@ -238,7 +242,11 @@ def bench_closure(cb, gen, ctl, max_iters, log, seed=0, legs=None, warm=True):
         out[name] = {"iterations": iters, "seconds": secs, "products": products,
                      "products_per_s": products / secs, "step_ms": [round(x, 2) for x in steps_ms],
                      "step_nnz": step_nnz, "fixed_point": st["changed"] == 0, "final_nnz": step_nnz[-1],
-                     "step_pool": step_mallocs}  # per step: hipMalloc calls, GB, cache drops
+                     # per step: hipMalloc calls, GB they allocated, cache drops, free and pool-cached
+                     # device GB at the step's start
+                     "step_pool": step_mallocs,
+                     # per step: ms of the SpGEMMs, ms of the unions, fallback rows, dumped dense rows
+                     "step_phases": step_phases}
         log(f"[bench] closure {name}: {iters} steps, {secs * 1e3:.1f} ms, {products / secs:.3e} products/s")
     # full-matrix digests of all 64 labels
     out["identical_results"] = len({tuple(v) for v in finals.values()}) == 1
@@ -599,6 +607,8 @@ def main():
                     help="run only the 64-label closure, up to this many steps or its fixed point, per-triple "
                          "plain against batched learned (each after a warm-up run of its own)")
     ap.add_argument("--closure-cold", action="store_true", help="with --closure-only: no warm-up runs")
+    ap.add_argument("--closure-legs", default="per_triple_plain,batched_learned",
+                    help="with --closure-only: comma-separated legs (per_triple_plain, batched_plain, batched_learned)")
     ap.add_argument("--c5-only", action="store_true",
                     help="run only the automaton (x) R-MAT 22/32 workload: the Kronecker in automaton-row "
                          "blocks and a bounded sample of one K * K closure step")
@@ -632,7 +642,7 @@ def main():
         ctl = cb.create_controls()
         train_predictor(cb, gen, ctl, log)
         print(json.dumps({"closure": bench_closure(cb, gen, ctl, args.closure_only, log,
-                                                   legs=("per_triple_plain", "batched_learned"),
+                                                   legs=tuple(args.closure_legs.split(",")),
                                                    warm=not args.closure_cold)}),
               flush=True)
         return

@@ -104,10 +104,11 @@ class Controls:
 
     def pool_stats(self) -> dict:
         """Buffer-pool diagnostics: cached bytes, hipMalloc calls and bytes,
-        cache drops after a failed hipMalloc."""
-        a = (C.c_uint64 * 4)()
+        cache drops after a failed hipMalloc, free and total device memory."""
+        a = (C.c_uint64 * 6)()
         _check(lib().clb_pool_stats(self.handle, a))
-        return {"cached_bytes": a[0], "mallocs": a[1], "malloc_bytes": a[2], "drops": a[3]}
+        return {"cached_bytes": a[0], "mallocs": a[1], "malloc_bytes": a[2], "drops": a[3],
+                "free_bytes": a[4], "total_bytes": a[5]}
 
     def trim(self):
         _check(lib().clb_pool_trim(self.handle))

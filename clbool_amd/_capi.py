@@ -92,7 +92,9 @@ class ClosureOpts(C.Structure):
 
 class ClosureStats(C.Structure):
     _fields_ = [("products", C.c_uint64), ("added", C.c_uint64), ("changed", C.c_uint32),
-                ("spgemm_calls", C.c_uint32), ("ms", C.c_double), ("max_block_bound", C.c_uint64)]
+                ("spgemm_calls", C.c_uint32), ("ms", C.c_double), ("max_block_bound", C.c_uint64),
+                ("ms_products", C.c_double), ("ms_unions", C.c_double), ("fallback_rows", C.c_uint64),
+                ("dense_rows", C.c_uint64)]
 
 
 CLOSURE_PER_TRIPLE = 0

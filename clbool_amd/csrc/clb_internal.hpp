@@ -17,11 +17,13 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <set>
 #include <stdexcept>
 #include <string>
 #include <vector>
 
 #include "../../include/clbool_c.h"
+#include "block_heap.hpp"
 
 namespace clb {
 
@@ -59,11 +61,20 @@ void check_hip(hipError_t e, Status s, const char *what);
 #define CLB_LAUNCHED(name) ::clb::check_hip(hipGetLastError(), ::clb::RUN_KERNEL_FAILURE, name)
 
 // ---------------------------------------------------------------------------
-// Per-device caching allocator.  Blocks are never returned to HIP while the
-// process runs unless an allocation fails (then the cache is dropped and the
-// allocation retried), so steady-state operations do no hipMalloc/hipFree.
+// Per-device caching allocator.  Memory comes from HIP in segments (at least
+// SEG_MIN bytes for large requests, SMALL_SEG for requests up to SMALL_MAX)
+// that the pool keeps; a request takes the smallest free block that holds it
+// and splits off the rest, and a released block merges with free neighbours
+// in its segment.  Segments go back to HIP only when they are entirely free
+// and an allocation fails (or on trim()), so after the first steps of a
+// workload the pool serves every size from memory it already holds: recycled
+// device memory makes hipMalloc slow (hundreds of milliseconds per call once
+// the device has freed much memory), whatever the request size.
 class Pool {
   public:
+    static constexpr u64 SMALL_MAX = 1ull << 20;  // requests up to 1 MiB: 256-byte units
+    static constexpr u64 SMALL_SEG = 2ull << 20;  // segments of the small requests
+    static constexpr u64 SEG_MIN = 1ull << 30;    // smallest segment of the large requests
     explicit Pool(int device) : device_(device) {}
     ~Pool();
     void *alloc(u64 bytes, u64 *cap_out);
@@ -71,24 +82,27 @@ class Pool {
     void trim();
     u64 cached_bytes() const {
         std::lock_guard<std::mutex> lk(mu_);
-        return cached_;
+        return small_.free_bytes() + large_.free_bytes();
     }
     int device() const { return device_; }
-    // [0] cached bytes, [1] hipMalloc calls, [2] bytes they allocated,
-    // [3] cache drops after a failed hipMalloc
+    // [0] free bytes inside the pool's segments, [1] hipMalloc calls, [2]
+    // bytes they allocated, [3] times free segments were handed back after a
+    // failed hipMalloc
     void stats(u64 out[4]) const {
         std::lock_guard<std::mutex> lk(mu_);
-        out[0] = cached_;
+        out[0] = small_.free_bytes() + large_.free_bytes();
         out[1] = mallocs_;
         out[2] = malloc_bytes_;
         out[3] = drops_;
     }
 
   private:
+    // hands entirely free segments back to HIP (counted in drops_ after a
+    // failed hipMalloc)
+    void drop_free_segments(bool after_failure);
     int device_;
     mutable std::mutex mu_;
-    std::multimap<u64, void *> free_;  // capacity -> pointer
-    u64 cached_ = 0;
+    BlockHeap small_{256}, large_{2ull << 20};
     u64 mallocs_ = 0, malloc_bytes_ = 0, drops_ = 0;
 };
 Pool &pool_for(int device);

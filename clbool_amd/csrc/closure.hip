@@ -231,6 +231,7 @@ clb_status clb_closure_step(clb_controls ctl, clb_dcsr *mats, uint32_t nmats, co
         auto xid = [&](u32 m) { return (u32)(std::lower_bound(xs.begin(), xs.end(), m) - xs.begin()); };
         clb_spgemm_opts so{};
         so.mode = o.spgemm_mode;
+        auto t_prod = t0;  // end of the products phase
 
         if (o.mode == CLB_CLOSURE_PER_TRIPLE) {
             // the reference-style loop: products from the step's start values,
@@ -241,8 +242,12 @@ clb_status clb_closure_step(clb_controls ctl, clb_dcsr *mats, uint32_t nmats, co
                     clb_spgemm_stats ss;
                     spg::spgemm_dispatch(ctl, mats[triples[3 * t + 1]], mats[triples[3 * t + 2]], so, prods[t], ss);
                     st.products += ss.products;
+                    st.fallback_rows += ss.fallback_rows;
+                    st.dense_rows += ss.dense_rows;
                     ++st.spgemm_calls;
                 }
+                sync(ctl);
+                t_prod = std::chrono::steady_clock::now();
                 for (u32 t = 0; t < ntriples; ++t) {
                     clb_dcsr &x = mats[triples[3 * t]];
                     clb_dcsr u{};
@@ -310,6 +315,8 @@ clb_status clb_closure_step(clb_controls ctl, clb_dcsr *mats, uint32_t nmats, co
                     DevArray<u32> sp(ctl, 2ull * (ntriples + 1));
                     auto absorb = [&](clb_dcsr &T, const clb_spgemm_stats &ss) {
                         st.products += ss.products;
+                        st.fallback_rows += ss.fallback_rows;
+                        st.dense_rows += ss.dense_rows;
                         ++st.spgemm_calls;
                         if (!T.nnz) {
                             clb_dcsr_release(&T);
@@ -369,6 +376,8 @@ clb_status clb_closure_step(clb_controls ctl, clb_dcsr *mats, uint32_t nmats, co
             }
             clb_dcsr_release(&Y);
             clb_dcsr_release(&Z);
+            sync(ctl);
+            t_prod = std::chrono::steady_clock::now();
             try {
                 // one union per distinct X.  Its keys and, per triple that
                 // targets it, the keys of that triple's product rows (its
@@ -454,7 +463,11 @@ clb_status clb_closure_step(clb_controls ctl, clb_dcsr *mats, uint32_t nmats, co
                 ++st.changed;
                 st.added += mats[k].nnz - before[k];
             }
-        st.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        sync(ctl);
+        const auto t1 = std::chrono::steady_clock::now();
+        st.ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+        st.ms_products = std::chrono::duration<double, std::milli>(t_prod - t0).count();
+        st.ms_unions = std::chrono::duration<double, std::milli>(t1 - t_prod).count();
         if (stats) *stats = st;
     });
 }

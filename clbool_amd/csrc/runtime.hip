@@ -29,40 +29,46 @@ void check_hip(hipError_t e, Status s, const char *what) {
 Pool::~Pool() { trim(); }
 
 static u64 round_capacity(u64 bytes) {
-    // 256-byte granularity for small blocks, 2 MiB for large ones, so that
-    // blocks of similar size are interchangeable in the cache.
-    if (bytes <= (1ull << 20)) return (bytes + 255) & ~255ull;
+    // 256-byte units for small blocks, 2 MiB for large ones
+    if (bytes <= Pool::SMALL_MAX) return (bytes + 255) & ~255ull;
     const u64 g = 2ull << 20;
     return (bytes + g - 1) / g * g;
 }
 
 void *Pool::alloc(u64 bytes, u64 *cap_out) {
     const u64 cap = round_capacity(bytes ? bytes : 1);
+    const bool small = cap <= SMALL_MAX;
+    BlockHeap &heap = small ? small_ : large_;
     {
         std::lock_guard<std::mutex> lk(mu_);
-        // best fit among cached blocks no more than 25% (+2 MiB) larger
-        auto it = free_.lower_bound(cap);
-        if (it != free_.end() && it->first <= cap + cap / 4 + (2ull << 20)) {
-            void *p = it->second;
-            *cap_out = it->first;
-            cached_ -= it->first;
-            free_.erase(it);
-            return p;
+        if (BlockHeap::Block *b = heap.take(cap)) {
+            *cap_out = b->size;
+            return b->ptr;
         }
     }
+    // a new segment: SEG_MIN or the request, whichever is larger (only the
+    // request itself when that much is not available)
+    const u64 seg = small ? SMALL_SEG : (cap > SEG_MIN ? cap : SEG_MIN);
     int prev = 0;
     CLB_HIP(hipGetDevice(&prev), CREATE_BUFFER_ERROR);
     if (prev != device_) CLB_HIP(hipSetDevice(device_), CREATE_BUFFER_ERROR);
     void *p = nullptr;
-    hipError_t e = hipMalloc(&p, cap);
+    u64 got = seg;
+    auto try_malloc = [&]() {
+        got = seg;
+        hipError_t e = hipMalloc(&p, got);
+        if (e != hipSuccess && seg > cap) {
+            (void)hipGetLastError();
+            got = cap;
+            e = hipMalloc(&p, got);
+        }
+        return e;
+    };
+    hipError_t e = try_malloc();
     if (e != hipSuccess) {
         (void)hipGetLastError();
-        trim();
-        {
-            std::lock_guard<std::mutex> lk(mu_);
-            ++drops_;
-        }
-        e = hipMalloc(&p, cap);
+        drop_free_segments(true);
+        e = try_malloc();
     }
     if (prev != device_) (void)hipSetDevice(prev);
     if (e != hipSuccess) {
@@ -71,37 +77,39 @@ void *Pool::alloc(u64 bytes, u64 *cap_out) {
         os << "device allocation of " << cap << " bytes failed: " << hipGetErrorString(e);
         raise(CREATE_BUFFER_ERROR, os.str());
     }
-    {
-        std::lock_guard<std::mutex> lk(mu_);
-        ++mallocs_;
-        malloc_bytes_ += cap;
-    }
-    *cap_out = cap;
-    return p;
+    std::lock_guard<std::mutex> lk(mu_);
+    ++mallocs_;
+    malloc_bytes_ += got;
+    heap.add_segment(static_cast<char *>(p), got);
+    BlockHeap::Block *b = heap.take(cap);  // fits: the new segment holds cap bytes
+    *cap_out = b->size;
+    return b->ptr;
 }
 
-void Pool::release(void *p, u64 cap) {
+void Pool::release(void *p, u64) {
     if (!p) return;
     std::lock_guard<std::mutex> lk(mu_);
-    free_.emplace(cap, p);
-    cached_ += cap;
+    if (!small_.give_back(static_cast<char *>(p))) (void)large_.give_back(static_cast<char *>(p));
 }
 
-void Pool::trim() {
-    std::multimap<u64, void *> drop;
+void Pool::drop_free_segments(bool after_failure) {
+    std::vector<std::pair<char *, u64>> drop;
     {
         std::lock_guard<std::mutex> lk(mu_);
-        drop.swap(free_);
-        cached_ = 0;
+        small_.collect_free_segments(drop);
+        large_.collect_free_segments(drop);
+        if (after_failure) ++drops_;
     }
     if (drop.empty()) return;
     int prev = 0;
     (void)hipGetDevice(&prev);
     (void)hipSetDevice(device_);
     (void)hipDeviceSynchronize();
-    for (auto &kv : drop) (void)hipFree(kv.second);
+    for (auto &kv : drop) (void)hipFree(kv.first);
     (void)hipSetDevice(prev);
 }
+
+void Pool::trim() { drop_free_segments(false); }
 
 Pool &pool_for(int device) {
     static std::mutex mu;
@@ -419,10 +427,19 @@ clb_status clb_last_stats(clb_controls ctl, clb_spgemm_stats *out) {
 
 clb_status clb_pool_trim(clb_controls ctl) { return guarded([&] { pool_for(ctl->device).trim(); }); }
 
-clb_status clb_pool_stats(clb_controls ctl, uint64_t out[4]) {
+clb_status clb_pool_stats(clb_controls ctl, uint64_t out[6]) {
     return guarded([&] {
         if (!ctl || !out) raise(INVALID_ARGUMENT, "null argument");
         pool_for(ctl->device).stats(out);
+        size_t free_b = 0, total_b = 0;
+        int prev = 0;
+        CLB_HIP(hipGetDevice(&prev), INVALID_ARGUMENT);
+        if (prev != ctl->device) CLB_HIP(hipSetDevice(ctl->device), INVALID_ARGUMENT);
+        const hipError_t e = hipMemGetInfo(&free_b, &total_b);
+        if (prev != ctl->device) (void)hipSetDevice(prev);
+        CLB_HIP(e, INVALID_ARGUMENT);
+        out[4] = free_b;
+        out[5] = total_b;
     });
 }
 

@@ -305,6 +305,8 @@ BinCfg make_bincfg(const clb_controls_s *ctl, u32 ncols) {
     while (wb < 15 && (1ull << wb) < words) ++wb;
     c.sym_bits = wb;
     c.bm_bits = (wb == 15 && words <= (1ull << 15)) ? 14 : wb;
+    const u32 forced = ctl->tuning.reserved[CLB_TUNE_BITMAP_WINDOW_BITS];
+    if (forced >= 13 && forced <= 15) c.bm_bits = forced < wb ? forced : wb;
     c.multi_slice = (u64)ncols > (32ull << c.bm_bits) ? 1u : 0u;
     const u32 em = ctl->tuning.reserved[CLB_TUNE_ESC_MAX_PRODUCTS];
     c.esc_max = em < (1u << ESC_MAX_BITS) ? em : (1u << ESC_MAX_BITS);

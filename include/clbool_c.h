@@ -109,7 +109,15 @@ typedef struct clb_tuning {
                                       reserved[CLB_TUNE_ONEPASS_BITMAP_MIN]: in
                                       the one pass, rows with at least this many
                                       products use a bitmap whatever their
-                                      predicted size (0: the size alone decides) */
+                                      predicted size (0: the size alone decides)
+                                      reserved[CLB_TUNE_BITMAP_WINDOW_BITS]:
+                                      13, 14 or 15: the numeric and one-pass
+                                      bitmap kernels use windows of 2^this
+                                      32-bit words (13 and 14: 512-thread
+                                      workgroups, two or more per CU; 15: one
+                                      1024-thread workgroup per CU), capped at
+                                      the matrix width; 0: 2^14 up to 2^20
+                                      columns, else 2^15 */
 } clb_tuning;
 #define CLB_TUNE_BITMAP_PRED_CAPS 0
 #define CLB_TUNE_GATHER_COMBINE   1
@@ -117,6 +125,7 @@ typedef struct clb_tuning {
 #define CLB_TUNE_COMPACT_TILES    3
 #define CLB_TUNE_HASH_WIDE        4
 #define CLB_TUNE_ONEPASS_BITMAP_MIN 5
+#define CLB_TUNE_BITMAP_WINDOW_BITS 6
 
 #define CLB_SPGEMM_PLAIN   0u  /* two-pass: symbolic count, scan, numeric      */
 #define CLB_SPGEMM_LEARNED 1u  /* one pass sized by the learned predictor,
@@ -205,8 +214,10 @@ clb_status clb_last_stats(clb_controls ctl, clb_spgemm_stats *out);
 clb_status clb_pool_trim(clb_controls ctl);
 /* Diagnostics of the device's buffer pool: out[0] bytes cached, out[1]
  * hipMalloc calls so far, out[2] bytes they allocated, out[3] times the cache
- * was dropped after a failed hipMalloc. */
-clb_status clb_pool_stats(clb_controls ctl, uint64_t out[4]);
+ * was dropped after a failed hipMalloc, out[4] free device memory now (as
+ * hipMemGetInfo reports it; cached blocks count as used), out[5] device
+ * memory in total. */
+clb_status clb_pool_stats(clb_controls ctl, uint64_t out[6]);
 
 /* ---- buffers (replace cl::Buffer) --------------------------------------- */
 /* A zero-size buffer fails with CLBOOL_CREATE_BUFFER_ERROR, as cl::Buffer
@@ -361,6 +372,10 @@ typedef struct clb_closure_stats {
     uint32_t spgemm_calls;
     double   ms;
     uint64_t max_block_bound; /* batched: largest output bound of a row block  */
+    double   ms_products;  /* host time of the SpGEMMs (stacking included)   */
+    double   ms_unions;    /* host time of the unions                        */
+    uint64_t fallback_rows; /* learned mode: rows re-run exactly, all blocks */
+    uint64_t dense_rows;   /* rows emitted from dumped bitmaps, all blocks   */
 } clb_closure_stats;
 /* One step X_t |= Y_t * Z_t for every triple t = (X, Y, Z) (indices into
  * mats, triples[3t..3t+2]); all products use the matrices as they were at

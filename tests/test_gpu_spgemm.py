@@ -104,6 +104,10 @@ def test_hypersparse_2p24(controls):
          gather_combine=1),
     dict(hash_max_slots=16384, bitmap_min_products=1 << 30, bitmap_slot_ratio=1, tiny_max_products=0,
          **{f"hash_wide_{b}": 1 for b in range(7, 14)}),
+    dict(hash_max_slots=16384, bitmap_min_products=1, bitmap_slot_ratio=1, tiny_max_products=0,
+         bitmap_window_bits=13),
+    dict(hash_max_slots=16384, bitmap_min_products=1, bitmap_slot_ratio=1, tiny_max_products=0,
+         bitmap_window_bits=15),
 ])
 def test_every_kernel_path(controls, knobs):
     """Force the rows through each bin family (hash only, tiny, bitmap only)."""

@@ -30,7 +30,8 @@ def test_oracle_suite_under_asan_and_ubsan():
     assert r.returncode == 0, r.stderr[-3000:]
     env = dict(os.environ)
     env.update({
-        "LD_PRELOAD": f"{asan}:{ubsan}",
+        # the sanitizer runtimes first, whatever the environment preloads after them
+        "LD_PRELOAD": ":".join([asan, ubsan] + ([os.environ["LD_PRELOAD"]] if os.environ.get("LD_PRELOAD") else [])),
         "ASAN_OPTIONS": "detect_leaks=0:abort_on_error=1",
         "UBSAN_OPTIONS": "halt_on_error=1:print_stacktrace=1",
         "CLBOOL_ORACLE_LIB": os.path.join(ROOT, "oracle", "_asan", "liboracle.so"),
