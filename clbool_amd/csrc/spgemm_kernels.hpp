@@ -253,33 +253,34 @@ __device__ __forceinline__ void for_each_product(const Mats &g, u32 js, u32 je, 
             // wave-uniform cursor: entry e holds windows [pre, next)
             u32 e = 0, pre = 0;
             u32 eb = lane_val(cbx, 0), ee = lane_val(cend, 0), next = lane_val(cpre, 1);
-            // loads window k into c; returns its number of columns (0: none)
+            // loads window k into c; returns its number of columns (0: none).
+            // Every call issues exactly UB loads (past a row's end, or past
+            // the last window, lanes re-read a column of the current entry,
+            // and apply masks them off): with a fixed number of loads in
+            // flight the compiler waits for one buffer only while the other
+            // buffer's loads are still outstanding, instead of draining all
+            // loads before each apply.
             auto fetch = [&](u32 k, u32(&c)[UB]) -> u32 {
-                if (k >= nwin) return 0u;
-                while (k >= next) {
-                    ++e;
-                    if (e + 1 - cbase >= (u32)dev::WAVE) load_cache(e);
-                    const u32 s = e - cbase;
-                    pre = next;
-                    eb = lane_val(cbx, s);
-                    ee = lane_val(cend, s);
-                    next = lane_val(cpre, s + 1);
-                }
-                const u32 start = eb + (k - pre) * WIN;
-                const u32 n = ee - start < WIN ? ee - start : WIN;
-                const u32 *src = g.b_cols + start;
-                if (n == WIN) {
-#pragma unroll
-                    for (int u = 0; u < UB; ++u) c[u] = src[u * 64 + l];
-                } else {
-                    // the row's last window: only its pieces are loaded; lanes
-                    // past its end re-read its last column and are masked off
-                    // when applied
-#pragma unroll
-                    for (int u = 0; u < UB; ++u) {
-                        const u32 o = (u32)(u * 64 + l);
-                        if ((u32)(u * 64) < n) c[u] = src[o < n ? o : n - 1];
+                u32 start = eb, n = 0;
+                if (k < nwin) {
+                    while (k >= next) {
+                        ++e;
+                        if (e + 1 - cbase >= (u32)dev::WAVE) load_cache(e);
+                        const u32 s = e - cbase;
+                        pre = next;
+                        eb = lane_val(cbx, s);
+                        ee = lane_val(cend, s);
+                        next = lane_val(cpre, s + 1);
                     }
+                    start = eb + (k - pre) * WIN;
+                    n = ee - start < WIN ? ee - start : WIN;
+                }
+                const u32 *src = g.b_cols + start;
+                const u32 last = n ? n - 1 : 0u;  // entries hold at least one column
+#pragma unroll
+                for (int u = 0; u < UB; ++u) {
+                    const u32 o = (u32)(u * 64 + l);
+                    c[u] = src[o < last ? o : last];
                 }
                 return n;
             };
