@@ -735,7 +735,9 @@ __device__ __forceinline__ void bitmap_row(const Mats &g, const u32 *__restrict_
             const u64 d = (u64)col - base;
             if (d < SPAN) atomicOr(&bm[(u32)(d >> 5)], 1u << (col & 31));
         };
-        // a slice read through the split table holds only its own columns
+        // a slice read through the split table holds only its own columns:
+        // no range check (base < 2^32 and a multiple of 32)
+        auto setu = [&](u32 col) { atomicOr(&bm[(col - (u32)base) >> 5], 1u << (col & 31)); };
         auto setdw = [&](u32 col) { dev::wave_or_sorted_bits(bm, (u32)((u64)col - base)); };
         constexpr int UBW = MODE == SYM ? 16 : 8;
         if (nslices == 1) {
@@ -744,10 +746,11 @@ __device__ __forceinline__ void bitmap_row(const Mats &g, const u32 *__restrict_
         } else {
             // only the slice's part of each B row is read (with the split table)
             if constexpr (COMB) {
-                if (use_splits) for_each_product<WG, UBW, LONG_ROW>(g, js, je, big, setd, setdw, s, sstep);
+                if (use_splits) for_each_product<WG, UBW, LONG_ROW>(g, js, je, big, setu, setdw, s, sstep);
                 else for_each_product<WG, UBW, LONG_ROW>(g, js, je, big, setd, NOSLICE, sstep);
             } else {
-                for_each_product<WG, UBW, LONG_ROW>(g, js, je, big, setd, use_splits ? s : NOSLICE, sstep);
+                if (use_splits) for_each_product<WG, UBW, LONG_ROW>(g, js, je, big, setu, s, sstep);
+                else for_each_product<WG, UBW, LONG_ROW>(g, js, je, big, setd, NOSLICE, sstep);
             }
         }
         __syncthreads();
