@@ -420,11 +420,10 @@ __device__ __forceinline__ u32 hash_insert(u32 *table, u32 col, u32 max_probe) {
     u32 h = dev::hash_slot(col, BITS);
     const u32 np = max_probe < T ? max_probe : T;
     for (u32 probe = 0; probe < np; ++probe) {
-        u32 cur = table[h];
-        if (cur == EMPTY) {
-            cur = atomicCAS(&table[h], EMPTY, col);
-            if (cur == EMPTY) return 1;
-        }
+        // one LDS round trip per probe: the compare-and-swap returns what
+        // the slot held (EMPTY: inserted; col: already there)
+        const u32 cur = atomicCAS(&table[h], EMPTY, col);
+        if (cur == EMPTY) return 1;
         if (cur == col) return 0;
         h = (h + 1) & (T - 1);
     }
