@@ -753,46 +753,6 @@ __device__ __forceinline__ void bitmap_row(const Mats &g, const u32 *__restrict_
             }
         }
         __syncthreads();
-        if (MODE != SYM && WBITS >= 14 && o.dump.store) {
-            // exact count per 1024-word chunk.  Slice 0 decides for the whole
-            // row: a dense row has every slice dumped (slice s at word s * W
-            // of its slot) and is written later by the emission kernel
-            constexpr u32 WPT = W / WG, TPC = EMIT_CHUNK / WPT, NCH = W / EMIT_CHUNK, Q = WPT / 4;
-            const uint4 *bm4 = reinterpret_cast<const uint4 *>(bm);
-            u32 cnt = 0;
-#pragma unroll 2
-            for (u32 q = 0; q < Q; ++q) {
-                // rotated start: neighbouring threads read different banks
-                const uint4 v = bm4[threadIdx.x * Q + ((q + threadIdx.x) & (Q - 1))];
-                cnt += __popc(v.x) + __popc(v.y) + __popc(v.z) + __popc(v.w);
-            }
-            u32 csum = cnt;
-#pragma unroll
-            for (u32 off2 = TPC / 2; off2 > 0; off2 >>= 1) csum += __shfl_xor(csum, off2, TPC);
-            const u32 n = dev::block_sum<u32, WG>(cnt, red);
-            // a one-pass row over its capacity already in its first window
-            // is dumped too: a dumped row needs no temporary slot
-            if (s == 0) {
-                dumping = n >= o.dump.dense_min || (MODE == ONE && n > lim);  // n is uniform
-                if (dumping) {
-                    // claim a slot (big is free between gathers)
-                    if (threadIdx.x == 0) big[0] = atomicAdd(o.dump.count, 1u);
-                    __syncthreads();
-                    slot = big[0];
-                    __syncthreads();
-                    dumping = slot < o.dump.capacity;
-                }
-            }
-            if (dumping) {
-                uint4 *dst = o.dump.store + ((u64)slot * nslices + s) * (W / 4);
-                for (u32 q = threadIdx.x; q < W / 4; q += WG) dst[q] = bm4[q];
-                if (threadIdx.x % TPC == 0)
-                    o.dump.chunk_cnt[((u64)slot * nslices + s) * NCH + threadIdx.x / TPC] = csum;
-                written += n;
-                __syncthreads();  // before the next slice clears the bitmap
-                continue;
-            }
-        }
         if (MODE == SYM) {
             const uint4 *bm4 = reinterpret_cast<const uint4 *>(bm);
             for (u32 t = threadIdx.x; t < W / 4; t += WG) {
@@ -800,16 +760,31 @@ __device__ __forceinline__ void bitmap_row(const Mats &g, const u32 *__restrict_
                 mine += __popc(v.x) + __popc(v.y) + __popc(v.z) + __popc(v.w);
             }
         } else {
-            constexpr u32 NW = WG / dev::WAVE, WPW = W / NW;
+            // One count pass per slice: thread t counts words [t * WPT,
+            // (t + 1) * WPT), so a wave's threads cover the words the wave
+            // emits, [wv * WPW, (wv + 1) * WPW), and TPC consecutive threads
+            // one 1024-word dump chunk.  One exchange of the wave totals
+            // gives the slice total (the dump decision) and each wave's
+            // offset (the emission).
+            constexpr u32 NW = WG / dev::WAVE, WPW = W / NW, WPT = W / WG;
             const int l = dev::lane();
             const u32 wv = dev::wave_id();
-            const u32 *wb = bm + wv * WPW;
             u32 cnt = 0;
-#pragma unroll 4
-            for (u32 q = l; q < WPW; q += dev::WAVE) cnt += __popc(wb[q]);
-            cnt = dev::wave_sum(cnt);
-            // per-wave totals -> wave offsets (red holds NW + 1 words)
-            if (l == 0) red[wv] = cnt;
+            if constexpr (WPT % 4 == 0) {
+                constexpr u32 Q = WPT / 4;
+                const uint4 *bm4 = reinterpret_cast<const uint4 *>(bm);
+#pragma unroll 2
+                for (u32 q = 0; q < Q; ++q) {
+                    // rotated start: neighbouring threads read different banks
+                    const uint4 v = bm4[threadIdx.x * Q + ((q + threadIdx.x) & (Q - 1))];
+                    cnt += __popc(v.x) + __popc(v.y) + __popc(v.z) + __popc(v.w);
+                }
+            } else {
+#pragma unroll
+                for (u32 q = 0; q < WPT; ++q) cnt += __popc(bm[threadIdx.x * WPT + q]);
+            }
+            const u32 wcnt = dev::wave_sum(cnt);
+            if (l == 0) red[wv] = wcnt;
             __syncthreads();
             u32 woff = 0, tot = 0;
             for (u32 v = 0; v < NW; ++v) {
@@ -818,6 +793,38 @@ __device__ __forceinline__ void bitmap_row(const Mats &g, const u32 *__restrict_
                 tot += t;
             }
             __syncthreads();
+            if constexpr (WBITS >= 14) if (o.dump.store) {
+                // Slice 0 decides for the whole row: a dense row has every
+                // slice dumped (slice s at word s * W of its slot) and is
+                // written later by the emission kernel.  A one-pass row over
+                // its capacity already in its first window is dumped too: a
+                // dumped row needs no temporary slot.
+                if (s == 0) {
+                    dumping = tot >= o.dump.dense_min || (MODE == ONE && tot > lim);  // tot is uniform
+                    if (dumping) {
+                        // claim a slot (big is free between gathers)
+                        if (threadIdx.x == 0) big[0] = atomicAdd(o.dump.count, 1u);
+                        __syncthreads();
+                        slot = big[0];
+                        __syncthreads();
+                        dumping = slot < o.dump.capacity;
+                    }
+                }
+                if (dumping) {
+                    constexpr u32 TPC = EMIT_CHUNK / WPT, NCH = W / EMIT_CHUNK;
+                    u32 csum = cnt;
+#pragma unroll
+                    for (u32 off2 = TPC / 2; off2 > 0; off2 >>= 1) csum += __shfl_xor(csum, off2, TPC);
+                    const uint4 *bm4 = reinterpret_cast<const uint4 *>(bm);
+                    uint4 *dst = o.dump.store + ((u64)slot * nslices + s) * (W / 4);
+                    for (u32 q = threadIdx.x; q < W / 4; q += WG) dst[q] = bm4[q];
+                    if (threadIdx.x % TPC == 0)
+                        o.dump.chunk_cnt[((u64)slot * nslices + s) * NCH + threadIdx.x / TPC] = csum;
+                    written += tot;
+                    __syncthreads();  // before the next slice clears the bitmap
+                    continue;
+                }
+            }
             if (MODE == ONE && written + tot > lim) {
                 // the capacity was too small: nothing of the row is kept
                 if (threadIdx.x == 0) {
@@ -826,6 +833,7 @@ __device__ __forceinline__ void bitmap_row(const Mats &g, const u32 *__restrict_
                 }
                 return;
             }
+            const u32 *wb = bm + wv * WPW;
             const u64 pos = written + woff;
             const u32 dense = g.emit ? g.emit : EMIT_DEFAULT;
             u64 done = 0;
