@@ -271,32 +271,56 @@ __global__ void k_failed_keys(const u32 *__restrict__ nnz_row, const u32 *__rest
     key2[i] = nnz_row[i] == FAILED ? (P < ncols ? P : ncols) : 0u;
 }
 
-// Copy each row from the temporary buffers to its exact place (one workgroup
-// per row) and accumulate the prediction error |log2(1+pred) - log2(1+n)|.
-__global__ __launch_bounds__(256) void k_compact_rows(const u32 *__restrict__ nnz_row, const u64 *__restrict__ off,
-                                                      const u32 *__restrict__ temp, const u64 *__restrict__ toff,
-                                                      const u32 *__restrict__ side, const u64 *__restrict__ soff,
-                                                      const u32 *__restrict__ key2, const u32 *__restrict__ isdense,
-                                                      const float *__restrict__ pred, u32 *__restrict__ cols,
-                                                      double *__restrict__ err) {
-    const u32 r = blockIdx.x;
-    const u32 n = nnz_row[r];
-    if (threadIdx.x == 0 && pred && n) {
-        const double e = fabs(log2(1.0 + (double)pred[r]) - log2(1.0 + (double)n));
-        atomicAdd(err, e);
+// Copy each row from the temporary buffers to its exact place and accumulate
+// the prediction error |log2(1+pred) - log2(1+n)|.  A workgroup takes
+// CR_ROWS consecutive rows: each wave copies one of them, except rows of at
+// least CR_LONG columns, which the whole workgroup copies in turn; the error
+// is reduced in LDS and added with one atomic per workgroup.
+constexpr u32 CR_WG = 256, CR_ROWS = CR_WG / 64, CR_LONG = 2048;
+__global__ __launch_bounds__(CR_WG) void k_compact_rows(const u32 *__restrict__ nnz_row, const u64 *__restrict__ off,
+                                                        u32 nrows, const u32 *__restrict__ temp,
+                                                        const u64 *__restrict__ toff, const u32 *__restrict__ side,
+                                                        const u64 *__restrict__ soff, const u32 *__restrict__ key2,
+                                                        const u32 *__restrict__ isdense, const float *__restrict__ pred,
+                                                        u32 *__restrict__ cols, double *__restrict__ err) {
+    __shared__ double red[CR_ROWS];
+    const u32 wv = dev::wave_id(), l = (u32)dev::lane();
+    const u32 r0 = blockIdx.x * CR_ROWS;
+    auto copy = [&](const u32 *src, u32 *dst, u32 n, u32 t, u32 step) {
+        for (; t + 3 * step < n; t += 4 * step) {
+            const u32 a = src[t], b = src[t + step], c = src[t + 2 * step], d = src[t + 3 * step];
+            dst[t] = a;
+            dst[t + step] = b;
+            dst[t + 2 * step] = c;
+            dst[t + 3 * step] = d;
+        }
+        for (; t < n; t += step) dst[t] = src[t];
+    };
+    for (u32 j = 0; j < CR_ROWS; ++j) {
+        const u32 r = r0 + j;
+        if (r >= nrows) break;  // uniform
+        const u32 n = nnz_row[r];
+        if (!n || (isdense && isdense[r])) continue;  // dense bitmap rows are emitted separately
+        const u32 *src = (key2 && key2[r]) ? side + soff[r] : temp + toff[r];
+        u32 *dst = cols + off[r];
+        if (n >= CR_LONG) copy(src, dst, n, threadIdx.x, CR_WG);
+        else if (wv == j) copy(src, dst, n, l, 64u);
     }
-    if (!n || (isdense && isdense[r])) return;  // dense bitmap rows are emitted separately
-    const u32 *src = (key2 && key2[r]) ? side + soff[r] : temp + toff[r];
-    u32 *dst = cols + off[r];
-    u32 t = threadIdx.x;
-    for (; t + 768 < n; t += 1024) {
-        const u32 a = src[t], b = src[t + 256], c = src[t + 512], d = src[t + 768];
-        dst[t] = a;
-        dst[t + 256] = b;
-        dst[t + 512] = c;
-        dst[t + 768] = d;
+    if (pred && err) {
+        double e = 0.0;
+        const u32 r = r0 + wv;
+        if (l == 0 && r < nrows) {
+            const u32 n = nnz_row[r];
+            if (n) e = fabs(log2(1.0 + (double)pred[r]) - log2(1.0 + (double)n));
+        }
+        if (l == 0) red[wv] = e;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double t = 0.0;
+            for (u32 k = 0; k < CR_ROWS; ++k) t += red[k];
+            if (t != 0.0) atomicAdd(err, t);
+        }
     }
-    for (; t < n; t += 256) dst[t] = src[t];
 }
 
 // The same copy cut by output position instead of by row: workgroup c writes
@@ -393,7 +417,8 @@ void compact_rows(hipStream_t st, u32 nrows, u64 nnz, const u32 *nnz_row, const 
                   const float *pred, u32 *cols, double *err, bool by_tiles) {
     if (!nrows) return;
     if (!by_tiles) {
-        k_compact_rows<<<nrows, 256, 0, st>>>(nnz_row, off, temp, toff, side, soff, key2, isdense, pred, cols, err);
+        k_compact_rows<<<div_up(nrows, CR_ROWS), CR_WG, 0, st>>>(nnz_row, off, nrows, temp, toff, side, soff, key2,
+                                                                  isdense, pred, cols, err);
         CLB_LAUNCHED("compact_rows");
         return;
     }
