@@ -524,8 +524,13 @@ static void launch_bitmap(clb_controls_s *ctl, hipStream_t st, const Setup &s, c
 void launch_bins(clb_controls_s *ctl, const Setup &s, const u32 *perm, const u32 off[NBINS + 1], int mode,
                  const Out &o, LaunchTimer *lt, int family) {
     fork_side(ctl);
+    // table kernels alternate over side streams 1.. (the first one takes
+    // stream 0 when no bitmap kernel was launched there)
     int k = 0;
-    auto next = [&]() { return side_stream(ctl, k++); };
+    auto next = [&]() {
+        constexpr int NS = clb_controls_s::NSIDE;
+        return k == 0 ? side_stream(ctl, k++) : side_stream(ctl, 1 + (k++ - 1) % (NS - 1));
+    };
     LaunchTimer dummy;
     LaunchTimer &t = lt ? *lt : dummy;
     // family = the pass (CLB_KP_*); < 0 picks symbolic / numeric from the mode
@@ -537,6 +542,40 @@ void launch_bins(clb_controls_s *ctl, const Setup &s, const u32 *perm, const u32
         else if (mode == NUM) KERNEL<__VA_ARGS__, NUM><<<GRID, WG, 0, ST>>>(s.g, pp, o);          \
         else KERNEL<__VA_ARGS__, ONE><<<GRID, WG, 0, ST>>>(s.g, pp, o);                           \
     } while (0)
+    // BITMAP (all classes in one launch, largest rows first; the ESC SpGEMM
+    // sorts these rows in global memory instead).  The longest kernel of the
+    // pass goes first and on a side stream of its own, so that it never
+    // queues behind a table kernel; the table kernels fill the CUs around it.
+    if (!s.cfg.esc_only) {
+        const u32 n = off[NBINS] - off[BIN_BITMAP0];
+        if (n) {
+            const u32 *pp = perm + off[BIN_BITMAP0];
+            hipStream_t st = side_stream(ctl, 0);
+            k = 1;
+            t.begin(st);
+#define BITMAP3(WB, WGS)                                                   \
+    do {                                                                   \
+        if (mode == SYM) launch_bitmap<WB, WGS, SYM>(ctl, st, s, pp, n, o); \
+        else if (mode == NUM) launch_bitmap<WB, WGS, NUM>(ctl, st, s, pp, n, o); \
+        else launch_bitmap<WB, WGS, ONE>(ctl, st, s, pp, n, o);          \
+    } while (0)
+            switch (mode == SYM ? s.cfg.sym_bits : s.cfg.bm_bits) {
+                case 6: BITMAP3(6, 64); break;
+                case 7: BITMAP3(7, 64); break;
+                case 8: BITMAP3(8, 64); break;
+                case 9: BITMAP3(9, 256); break;
+                case 10: BITMAP3(10, 256); break;
+                case 11: BITMAP3(11, 256); break;
+                case 12: BITMAP3(12, 512); break;
+                case 13: BITMAP3(13, 512); break;
+                case 14: BITMAP3(14, 512); break;
+                case 15: BITMAP3(15, 1024); break;
+            }
+#undef BITMAP3
+            CLB_LAUNCHED("bitmap");
+            t.end(fam(2), st);
+        }
+    }
     // TINY
     {
         const u32 n = off[BIN_TINY + 1] - off[BIN_TINY];
@@ -598,37 +637,6 @@ void launch_bins(clb_controls_s *ctl, const Setup &s, const u32 *perm, const u32
         }
         CLB_LAUNCHED("esc");
         t.end(fam(1), st);
-    }
-    // BITMAP (all classes in one launch, largest rows first; the ESC SpGEMM
-    // sorts these rows in global memory instead)
-    if (!s.cfg.esc_only) {
-        const u32 n = off[NBINS] - off[BIN_BITMAP0];
-        if (n) {
-            const u32 *pp = perm + off[BIN_BITMAP0];
-            hipStream_t st = next();
-            t.begin(st);
-#define BITMAP3(WB, WGS)                                                   \
-    do {                                                                   \
-        if (mode == SYM) launch_bitmap<WB, WGS, SYM>(ctl, st, s, pp, n, o); \
-        else if (mode == NUM) launch_bitmap<WB, WGS, NUM>(ctl, st, s, pp, n, o); \
-        else launch_bitmap<WB, WGS, ONE>(ctl, st, s, pp, n, o);          \
-    } while (0)
-            switch (mode == SYM ? s.cfg.sym_bits : s.cfg.bm_bits) {
-                case 6: BITMAP3(6, 64); break;
-                case 7: BITMAP3(7, 64); break;
-                case 8: BITMAP3(8, 64); break;
-                case 9: BITMAP3(9, 256); break;
-                case 10: BITMAP3(10, 256); break;
-                case 11: BITMAP3(11, 256); break;
-                case 12: BITMAP3(12, 512); break;
-                case 13: BITMAP3(13, 512); break;
-                case 14: BITMAP3(14, 512); break;
-                case 15: BITMAP3(15, 1024); break;
-            }
-#undef BITMAP3
-            CLB_LAUNCHED("bitmap");
-            t.end(fam(2), st);
-        }
     }
 #undef DISPATCH3
     join_side(ctl);
