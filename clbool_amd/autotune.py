@@ -34,13 +34,14 @@ KNOBS = {
     "compact_tiles": [0, 1],      # one pass: compaction by output tiles (1) or one workgroup per row (0)
     "onepass_bitmap_min": [0, 8192, 32768, 131072],  # one pass: product count that forces a bitmap (0: never)
     "bitmap_window_bits": [0, 13, 14, 15],  # numeric / one-pass bitmap window, 2^bits words (0: by width)
+    "emit_sparse_bits": [0, 64, 1024, 4096],  # bitmap slices below this many bits per 1024 words: lane-own emission
 }
 # workgroup shape of each LDS hash table size: default (0) or twice the threads (1)
 for _b in range(7, 14):
     KNOBS[f"hash_wide_{_b}"] = [0, 1]
 # knobs that live in clb_tuning.reserved[] (whole words, or one bit of a word)
 RESERVED = {"bitmap_pred_caps": 0, "gather_combine": 1, "esc_max_products": 2, "compact_tiles": 3,
-            "onepass_bitmap_min": 5, "bitmap_window_bits": 6}
+            "onepass_bitmap_min": 5, "bitmap_window_bits": 6, "emit_sparse_bits": 7}
 RESERVED_BITS = {f"hash_wide_{b}": (4, b - 7) for b in range(7, 14)}
 LEARNED_ONLY = ("learned_margin_pct", "bitmap_pred_caps")   # knobs of the predictor's sizing
 ONE_PASS_ONLY = ("compact_tiles", "onepass_bitmap_min")      # knobs of the learned and bound one pass

@@ -255,15 +255,14 @@ __device__ __forceinline__ uint32_t emit_step(uint32_t word, uint32_t cb, uint64
     return st;
 }
 
-// emit_step with the wave's columns staged in LDS: each lane writes its
-// word's columns into the wave's staging area `sg` (STAGE_WORDS words), then
-// the wave copies them out in order, 64 consecutive columns per store
-// instruction.  Scattered per-lane stores would each touch their own cache
-// line; the staged copy writes whole lines.  In a dense step (more than four
-// bits per word on average) lanes write about 16 apart, which would put all
-// 64 lanes on a few LDS banks, so column t then sits at sg[t + t / 32] (one
-// padding word per 32); sparser steps stage unpadded.  Returns the step's
-// set bits.
+// Staged emission: each lane writes its words' columns into the wave's
+// staging area `sg` (STAGE_WORDS words), then the wave copies them out in
+// order, 64 consecutive columns per store instruction.  Scattered per-lane
+// stores would each touch their own cache line; the staged copy writes whole
+// lines.  When the columns average more than four per lane, lanes write
+// about 16 apart, which would put all 64 lanes on a few LDS banks, so column
+// t then sits at sg[t + t / 32] (one padding word per 32); sparser rounds
+// stage unpadded.
 constexpr uint32_t STAGE_WORDS = 2048 + 64;
 // Lane l copies columns l, l + 64, ... of n: column 64k + l sits at src[KS * k]
 // (src = the lane's first slot; KS = 66 padded, 64 unpadded) and goes to byte
@@ -283,42 +282,61 @@ __device__ __forceinline__ void staged_copy_out(const uint32_t *src, char *dst, 
     if (WAVE * k + l < n) *reinterpret_cast<uint32_t *>(dst + b) = src[KS * k];
 }
 
-__device__ __forceinline__ uint32_t emit_step_staged(uint32_t word, uint32_t cb, uint64_t pos, uint32_t *out,
-                                                     uint64_t lim, uint32_t *sg) {
+// Writes the set bits of 256 bitmap words as columns at out[pos ...] (never
+// at or past lim): lane l holds words 4l .. 4l + 3 (x[0..3]; bit b of x[e]
+// is column cb + 32 * (4l + e) + b).  One wave scan per round; a round whose
+// columns fit the staging area goes through it, a denser one is written two
+// words at a time with a lane per bit (64 consecutive columns per ballot,
+// stores in order).  Returns the round's set bits.
+__device__ __forceinline__ uint32_t emit_round4_staged(const uint32_t (&x)[4], uint32_t cb, uint64_t pos,
+                                                       uint32_t *out, uint64_t lim, uint32_t *sg) {
     const uint32_t l = (uint32_t)lane();
-    const uint32_t pc = (uint32_t)__popc(word);
+    const uint32_t pc = (uint32_t)(__popc(x[0]) + __popc(x[1]) + __popc(x[2]) + __popc(x[3]));
     const uint32_t inc = wave_incl_sum_dpp(pc);
     const uint32_t st = (uint32_t)__builtin_amdgcn_readlane((int)inc, 63);
     if (st == 0) return 0;
-    uint32_t w = word;
-    const uint32_t o = inc - pc;
-    const uint32_t c0 = cb + 32u * l;
-    const bool padded = st > 4u * WAVE;  // uniform
-    if (padded) {
-        uint32_t q = o;
-        while (w) {
-            sg[q + (q >> 5)] = c0 + (uint32_t)__builtin_ctz(w);
-            ++q;
-            w &= w - 1u;
-        }
-    } else {
-        uint32_t *p = sg + o;
-        while (w) {
-            *p++ = c0 + (uint32_t)__builtin_ctz(w);
-            w &= w - 1u;
-        }
-    }
-    __builtin_amdgcn_wave_barrier();
-    // the row's base as a uniform (scalar) pointer: stores take a 32-bit lane offset
     const uint64_t up = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(pos >> 32)) << 32) |
                         (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)pos);
     char *dst = reinterpret_cast<char *>(out + up);
-    // a row holds fewer than 2^32 columns
     const uint32_t room = (uint32_t)(lim > up ? lim - up : 0);
     const uint32_t n = st < room ? st : room;
-    if (padded) staged_copy_out<66>(sg + l + (l >> 5), dst, l, n);
-    else staged_copy_out<64>(sg + l, dst, l, n);
-    __builtin_amdgcn_wave_barrier();
+    if (st <= STAGE_WORDS - WAVE) {
+        const bool padded = st > 4u * WAVE;  // uniform
+        uint32_t q = inc - pc;
+#pragma unroll
+        for (uint32_t e = 0; e < 4; ++e) {
+            uint32_t w = x[e];
+            const uint32_t c0 = cb + 32u * (4u * l + e);
+            while (w) {
+                sg[padded ? q + (q >> 5) : q] = c0 + (uint32_t)__builtin_ctz(w);
+                ++q;
+                w &= w - 1u;
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        if (padded) staged_copy_out<66>(sg + l + (l >> 5), dst, l, n);
+        else staged_copy_out<64>(sg + l, dst, l, n);
+        __builtin_amdgcn_wave_barrier();
+    } else {
+        // words 4j + 2h and 4j + 2h + 1 of lane j: lanes 0..31 take the
+        // first word's bits, lanes 32..63 the second's
+        const uint32_t sh = l & 31u;
+        uint32_t p = 0;
+        for (uint32_t j = 0; j < WAVE; ++j) {
+#pragma unroll
+            for (uint32_t h = 0; h < 2; ++h) {
+                const uint32_t w0 = (uint32_t)__builtin_amdgcn_readlane((int)x[2 * h], (int)j);
+                const uint32_t w1 = (uint32_t)__builtin_amdgcn_readlane((int)x[2 * h + 1], (int)j);
+                if ((w0 | w1) == 0) continue;
+                const uint32_t v = l < 32 ? w0 : w1;
+                const bool set = (v >> sh) & 1u;
+                const uint64_t m = __ballot(set);
+                const uint32_t at = p + lanes_below(m);
+                if (set && at < n) *reinterpret_cast<uint32_t *>(dst + 4u * at) = cb + 32u * (4u * j + 2u * h) + l;
+                p += (uint32_t)__popcll(m);
+            }
+        }
+    }
     return st;
 }
 

@@ -136,10 +136,11 @@ __global__ void k_set_positions(const u32 *__restrict__ nnz_row, const u64 *__re
 // their final positions.  Each of the 2 waves owns a contiguous half of
 // the row's words; its start position is the prefix of the chunk counts
 // before its first chunk, so the waves need no exchange.  A wave loads its
-// words 16 steps at a time, double-buffered (the next 16 steps' loads are in
-// flight while the current ones are expanded), and writes them through its
-// own LDS staging area (dev::emit_step_staged), so every store instruction
-// is one contiguous run of columns.  Two-wave workgroups (8.4 KB of staging
+// words 1024 at a time (four rounds of 256, 16 bytes per lane per round),
+// double-buffered (the next 1024 words' loads are in flight while the
+// current ones are expanded), and writes them through its own LDS staging
+// area (dev::emit_round4_staged: one wave scan per round), so every store
+// instruction is one contiguous run of columns.  Two-wave workgroups (8.4 KB of staging
 // per wave) let nine rows share a CU, so one row's load latency overlaps
 // another's expansion (at 256 threads, four rows: 11% slower).
 // gpre = exclusive prefix of chunk_cnt over all slots.
@@ -148,7 +149,7 @@ __global__ __launch_bounds__(EMIT_WG) void k_bitmap_emit(const uint4 *__restrict
                                                          const u64 *__restrict__ gpre, const u32 *__restrict__ list,
                                                          const u32 *__restrict__ perm, const u64 *__restrict__ off,
                                                          u32 words_per_row, u32 *__restrict__ cols) {
-    constexpr u32 NW = EMIT_WG / 64, BATCH = 16, SPAN = BATCH * 64;
+    constexpr u32 NW = EMIT_WG / 64, ROUNDS = 4, SPAN = ROUNDS * 256;
     __shared__ u32 stage[NW][dev::STAGE_WORDS];
     const u32 slot = blockIdx.x;  // slots are claimed densely from 0
     const u32 r = perm[list[slot]];
@@ -161,16 +162,24 @@ __global__ __launch_bounds__(EMIT_WG) void k_bitmap_emit(const uint4 *__restrict
     const u32 first = wv * wpw;
     const u32 *words = reinterpret_cast<const u32 *>(store) + (u64)slot * words_per_row + first;
     u64 p = gpre[(u64)slot * nch + first / EMIT_CHUNK] - gpre[(u64)slot * nch];
-    auto load = [&](u32(&w)[BATCH], u32 s0) {
+    // a batch is ROUNDS rounds of 256 words, lane l holding words 4l .. 4l + 3
+    // of each (16-byte loads)
+    auto load = [&](u32(&w)[ROUNDS][4], u32 s0) {
 #pragma unroll
-        for (u32 k = 0; k < BATCH; ++k) w[k] = words[s0 + k * 64 + l];
+        for (u32 k = 0; k < ROUNDS; ++k) {
+            const uint4 v = *reinterpret_cast<const uint4 *>(words + s0 + k * 256 + 4 * l);
+            w[k][0] = v.x;
+            w[k][1] = v.y;
+            w[k][2] = v.z;
+            w[k][3] = v.w;
+        }
     };
-    auto expand = [&](const u32(&w)[BATCH], u32 s0) {
+    auto expand = [&](const u32(&w)[ROUNDS][4], u32 s0) {
 #pragma unroll
-        for (u32 k = 0; k < BATCH; ++k)
-            p += dev::emit_step_staged(w[k], (first + s0 + k * 64) * 32u, p, out, lim, stage[wv]);
+        for (u32 k = 0; k < ROUNDS; ++k)
+            p += dev::emit_round4_staged(w[k], (first + s0 + k * 256) * 32u, p, out, lim, stage[wv]);
     };
-    u32 wa[BATCH], wb[BATCH];
+    u32 wa[ROUNDS][4], wb[ROUNDS][4];
     load(wa, 0);
     for (u32 s0 = 0; s0 < wpw; s0 += 2 * SPAN) {
         load(wb, s0 + SPAN);
@@ -369,6 +378,8 @@ void prepare(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &b, u32 row_
         s.g.mapn = s.mapn;
         s.g.ncols = b.ncols;
         s.g.emit = ctl->tuning.emit_lane_bits_min;
+        const u32 sb = ctl->tuning.reserved[CLB_TUNE_EMIT_SPARSE_BITS];
+        s.g.sparse_bits = sb ? sb : 256u;
         s.g.combine = ctl->tuning.reserved[CLB_TUNE_GATHER_COMBINE];
         s.cfg = make_bincfg(ctl, b.ncols);
         s.g.splits = nullptr;

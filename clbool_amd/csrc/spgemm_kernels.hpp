@@ -49,6 +49,7 @@ struct Mats {
     u32 row0;             // first A stored row of the block
     u32 ncols;            // columns of B (and C)
     u32 emit;             // bitmap emission: lane-per-bit threshold (EMIT_DEFAULT when 0)
+    u32 sparse_bits;      // bitmap emission: set bits per 1024 words below which lanes write their own words
     u32 combine;          // bitmap gather: OR bits of one word within a wave before the LDS atomic
     // Column slices of the bitmap kernels (nsl > 1 when ncols exceeds one
     // window of 2^split_bits words): splits[k * (nsl + 1) + s] is the
@@ -760,28 +761,46 @@ __device__ __forceinline__ void bitmap_row(const Mats &g, const u32 *__restrict_
                 mine += __popc(v.x) + __popc(v.y) + __popc(v.z) + __popc(v.w);
             }
         } else {
-            // One count pass per slice: thread t counts words [t * WPT,
-            // (t + 1) * WPT), so a wave's threads cover the words the wave
-            // emits, [wv * WPW, (wv + 1) * WPW), and TPC consecutive threads
-            // one 1024-word dump chunk.  One exchange of the wave totals
-            // gives the slice total (the dump decision) and each wave's
-            // offset (the emission).
-            constexpr u32 NW = WG / dev::WAVE, WPW = W / NW, WPT = W / WG;
+            // Each wave owns words [wv * WPW, (wv + 1) * WPW) of the window: in
+            // round m, lane l takes the E consecutive words starting at
+            // (m * 64 + l) * E (16-byte reads by consecutive lanes, no bank
+            // conflicts).  One read of every round gives the counts (one
+            // exchange of the wave totals gives the slice total for the dump
+            // decision and each wave's offset); the dump re-reads every
+            // round, the emission only the rounds that hold bits, with one
+            // wave scan per round and each lane writing its own words'
+            // columns.
+            constexpr u32 NW = WG / dev::WAVE, WPW = W / NW, LPW = WPW / dev::WAVE;
+            constexpr u32 E = LPW >= 4 ? 4 : LPW, R = LPW / E;
             const int l = dev::lane();
             const u32 wv = dev::wave_id();
-            u32 cnt = 0;
-            if constexpr (WPT % 4 == 0) {
-                constexpr u32 Q = WPT / 4;
-                const uint4 *bm4 = reinterpret_cast<const uint4 *>(bm);
-#pragma unroll 2
-                for (u32 q = 0; q < Q; ++q) {
-                    // rotated start: neighbouring threads read different banks
-                    const uint4 v = bm4[threadIdx.x * Q + ((q + threadIdx.x) & (Q - 1))];
-                    cnt += __popc(v.x) + __popc(v.y) + __popc(v.z) + __popc(v.w);
+            u32 *wb = bm + wv * WPW;
+            auto words = [&](u32 m, u32(&x)[E]) {
+                const u32 i0 = (m * dev::WAVE + (u32)l) * E;
+                if constexpr (E == 4) {
+                    const uint4 v = *reinterpret_cast<const uint4 *>(wb + i0);
+                    x[0] = v.x;
+                    x[1] = v.y;
+                    x[2] = v.z;
+                    x[3] = v.w;
+                } else if constexpr (E == 2) {
+                    const uint2 v = *reinterpret_cast<const uint2 *>(wb + i0);
+                    x[0] = v.x;
+                    x[1] = v.y;
+                } else {
+                    x[0] = wb[i0];
                 }
-            } else {
+            };
+            u32 c[R];
+            u32 cnt = 0;
 #pragma unroll
-                for (u32 q = 0; q < WPT; ++q) cnt += __popc(bm[threadIdx.x * WPT + q]);
+            for (u32 m = 0; m < R; ++m) {
+                u32 x[E];
+                words(m, x);
+                c[m] = 0;
+#pragma unroll
+                for (u32 e = 0; e < E; ++e) c[m] += __popc(x[e]);
+                cnt += c[m];
             }
             const u32 wcnt = dev::wave_sum(cnt);
             if (l == 0) red[wv] = wcnt;
@@ -811,15 +830,22 @@ __device__ __forceinline__ void bitmap_row(const Mats &g, const u32 *__restrict_
                     }
                 }
                 if (dumping) {
-                    constexpr u32 TPC = EMIT_CHUNK / WPT, NCH = W / EMIT_CHUNK;
-                    u32 csum = cnt;
+                    // a wave's range holds CPW dump chunks of RPC rounds each
+                    constexpr u32 NCH = W / EMIT_CHUNK, CPW = WPW / EMIT_CHUNK, RPC = R / CPW;
+                    static_assert(E == 4 && CPW * EMIT_CHUNK == WPW && RPC * CPW == R, "dump chunk layout");
+                    const u64 sl = (u64)slot * nslices + s;
 #pragma unroll
-                    for (u32 off2 = TPC / 2; off2 > 0; off2 >>= 1) csum += __shfl_xor(csum, off2, TPC);
-                    const uint4 *bm4 = reinterpret_cast<const uint4 *>(bm);
-                    uint4 *dst = o.dump.store + ((u64)slot * nslices + s) * (W / 4);
-                    for (u32 q = threadIdx.x; q < W / 4; q += WG) dst[q] = bm4[q];
-                    if (threadIdx.x % TPC == 0)
-                        o.dump.chunk_cnt[((u64)slot * nslices + s) * NCH + threadIdx.x / TPC] = csum;
+                    for (u32 k = 0; k < CPW; ++k) {
+                        u32 cc = 0;
+#pragma unroll
+                        for (u32 m = k * RPC; m < (k + 1) * RPC; ++m) cc += c[m];
+                        cc = dev::wave_sum(cc);
+                        if (l == 0) o.dump.chunk_cnt[sl * NCH + wv * CPW + k] = cc;
+                    }
+                    uint4 *dst = o.dump.store + sl * (W / 4) + wv * (WPW / 4);
+                    const uint4 *src4 = reinterpret_cast<const uint4 *>(wb);
+#pragma unroll 4
+                    for (u32 m = 0; m < R; ++m) dst[m * dev::WAVE + (u32)l] = src4[m * dev::WAVE + (u32)l];
                     written += tot;
                     __syncthreads();  // before the next slice clears the bitmap
                     continue;
@@ -833,13 +859,41 @@ __device__ __forceinline__ void bitmap_row(const Mats &g, const u32 *__restrict_
                 }
                 return;
             }
-            const u32 *wb = bm + wv * WPW;
-            const u64 pos = written + woff;
-            const u32 dense = g.emit ? g.emit : EMIT_DEFAULT;
-            u64 done = 0;
-            for (u32 q0 = 0; q0 < WPW; q0 += dev::WAVE) {
-                const u32 word = wb[q0 + l];
-                done += dev::emit_step(word, (u32)(base + 32ull * (wv * WPW + q0)), pos + done, out, lim, dense);
+            if ((u64)tot * 1024u >= (u64)g.sparse_bits * W) {
+                // a denser slice: 64 words per step (their stores land close
+                // together), a lane per bit where a step is dense enough
+                const u64 pos = written + woff;
+                const u32 dense = g.emit ? g.emit : EMIT_DEFAULT;
+                u64 done = 0;
+                for (u32 q0 = 0; q0 < WPW; q0 += dev::WAVE) {
+                    const u32 word = wb[q0 + l];
+                    done += dev::emit_step(word, (u32)(base + 32ull * (wv * WPW + q0)), pos + done, out, lim, dense);
+                }
+            } else {
+                u64 pos = written + woff;
+                const u32 colw = (u32)(base + 32ull * wv * WPW);  // column of the wave's first word
+#pragma unroll
+                for (u32 m = 0; m < R; ++m) {
+                    const u32 inc = dev::wave_incl_sum_dpp(c[m]);
+                    const u32 rt = (u32)__builtin_amdgcn_readlane((int)inc, 63);
+                    if (rt == 0) continue;  // uniform
+                    u64 q = pos + (inc - c[m]);
+                    const u32 cw = colw + 32u * ((m * dev::WAVE + (u32)l) * E);
+                    u32 x[E];
+                    if (c[m]) words(m, x);
+#pragma unroll
+                    for (u32 e = 0; e < E; ++e) {
+                        u32 w = c[m] ? x[e] : 0u;
+                        while (w) {
+                            // the numeric pass's slot is exact; a count that
+                            // disagrees never writes past it
+                            if (MODE == ONE || q < lim) out[q] = cw + 32u * e + (u32)__builtin_ctz(w);
+                            ++q;
+                            w &= w - 1u;
+                        }
+                    }
+                    pos += rt;
+                }
             }
             written += tot;
         }

@@ -117,7 +117,13 @@ typedef struct clb_tuning {
                                       workgroups, two or more per CU; 15: one
                                       1024-thread workgroup per CU), capped at
                                       the matrix width; 0: 2^14 up to 2^20
-                                      columns, else 2^15 */
+                                      columns, else 2^15
+                                      reserved[CLB_TUNE_EMIT_SPARSE_BITS]: a
+                                      bitmap window slice with fewer set bits
+                                      than this per 1024 words is written by
+                                      each lane from its own words (one wave
+                                      scan per 256 words, empty ones skipped);
+                                      denser slices 64 words per step (0: 256) */
 } clb_tuning;
 #define CLB_TUNE_BITMAP_PRED_CAPS 0
 #define CLB_TUNE_GATHER_COMBINE   1
@@ -126,6 +132,7 @@ typedef struct clb_tuning {
 #define CLB_TUNE_HASH_WIDE        4
 #define CLB_TUNE_ONEPASS_BITMAP_MIN 5
 #define CLB_TUNE_BITMAP_WINDOW_BITS 6
+#define CLB_TUNE_EMIT_SPARSE_BITS   7
 
 #define CLB_SPGEMM_PLAIN   0u  /* two-pass: symbolic count, scan, numeric      */
 #define CLB_SPGEMM_LEARNED 1u  /* one pass sized by the learned predictor,

@@ -92,16 +92,18 @@ def test_learned_plan_recuts_blocks_over_the_bound(controls):
     assert nb > planned
 
 
-@pytest.mark.parametrize("bitmap_pred,combine,window", [(0, 0, 0), (1, 0, 0), (1, 1, 0), (1, 0, 13), (0, 0, 14),
-                                                         (1, 0, 15)])
-def test_bound_mode_and_bitmap_caps_exact(controls, bitmap_pred, combine, window):
+@pytest.mark.parametrize("bitmap_pred,combine,window,sparse", [(0, 0, 0, 0), (1, 0, 0, 0), (1, 1, 0, 0), (1, 0, 13, 0),
+                                                                (0, 0, 14, 0), (1, 0, 15, 0), (1, 0, 0, 1),
+                                                                (0, 0, 0, 1 << 20)])
+def test_bound_mode_and_bitmap_caps_exact(controls, bitmap_pred, combine, window, sparse):
     # products whose rows reach the bitmap kernels (2^12..2^14 columns) and a
     # wide product (2^21 columns: bitmap rows in several column windows; 2, 4
-    # or 8 of them with 2^15, 2^14 or 2^13-word windows)
+    # or 8 of them with 2^15, 2^14 or 2^13-word windows); sparse 1 / 2^20:
+    # every window slice emitted 64 words per step / by each lane's own words
     base = at.tuning_dict(controls.tuning())
     try:
         at.apply(controls, {**base, "bitmap_pred_caps": bitmap_pred, "gather_combine": combine,
-                            "bitmap_window_bits": window})
+                            "bitmap_window_bits": window, "emit_sparse_bits": sparse})
         p = np.zeros(cb.MLP_PARAMS, np.float32)
         p[-1] = 9.0 / 32.0  # predicts 511 columns for every row
         cb.predictor_set_params(controls, p)

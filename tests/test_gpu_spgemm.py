@@ -108,6 +108,10 @@ def test_hypersparse_2p24(controls):
          bitmap_window_bits=13),
     dict(hash_max_slots=16384, bitmap_min_products=1, bitmap_slot_ratio=1, tiny_max_products=0,
          bitmap_window_bits=15),
+    dict(hash_max_slots=16384, bitmap_min_products=1, bitmap_slot_ratio=1, tiny_max_products=0,
+         emit_sparse_bits=1),
+    dict(hash_max_slots=16384, bitmap_min_products=1, bitmap_slot_ratio=1, tiny_max_products=0,
+         emit_sparse_bits=1 << 20, dump_bits_per_chunk=65536),
 ])
 def test_every_kernel_path(controls, knobs):
     """Force the rows through each bin family (hash only, tiny, bitmap only)."""
