@@ -17,6 +17,10 @@ also carries the roofline of the dominant kernel (algorithmic bytes per
 launch over its HIP-event duration) and the CPU baseline (the reference test
 suite's host SpGEMM, matrix_multiplication_cpu restated in oracle/, timed on
 a bounded row sample) beside a multi-threaded Gustavson on the host cores.
+Reported beside the main metric as well: the 64-label closure (4 steps; to
+its fixed point with --closure-only), the hypersparse learned-sizing workload
+(2^24 x 2^24, the predictor trained on 200 instances) and the automaton (x)
+R-MAT 22/32 Kronecker with one K * K step at G scale 16.
 
     python bench.py [--gpus 1] [--steps 5] [--warmup 2] [--workload rmat22] [--also rmat20]
 """
@@ -599,7 +603,7 @@ def main():
                     help="override a tuning knob for every mode (after the tuning file)")
     ap.add_argument("--autotune", action="store_true", help="tune the knobs on the workload first")
     ap.add_argument("--save-tuning", default=None)
-    ap.add_argument("--hypersparse", type=int, default=0,
+    ap.add_argument("--hypersparse", type=int, default=200,
                     help="also run the hypersparse learned-sizing workload, training on this many instances")
     ap.add_argument("--hypersparse-only", type=int, default=0,
                     help="run only the hypersparse workload, training on this many instances")
@@ -609,6 +613,8 @@ def main():
     ap.add_argument("--closure-cold", action="store_true", help="with --closure-only: no warm-up runs")
     ap.add_argument("--closure-legs", default="per_triple_plain,batched_learned",
                     help="with --closure-only: comma-separated legs (per_triple_plain, batched_plain, batched_learned)")
+    ap.add_argument("--no-c5", action="store_true",
+                    help="skip the automaton (x) R-MAT 22/32 workload reported beside the main metric")
     ap.add_argument("--c5-only", action="store_true",
                     help="run only the automaton (x) R-MAT 22/32 workload: the Kronecker in automaton-row "
                          "blocks and a bounded sample of one K * K closure step")
@@ -725,6 +731,10 @@ def main():
         out["closure"] = bench_closure(cb, gen, ctl, args.closure_iters, log)
     if args.hypersparse > 0:
         out["hypersparse"] = bench_hypersparse(cb, gen, ctl, args.hypersparse, log)
+    if not args.no_c5 and "learned" in modes:
+        ctl.trim()
+        at.apply(ctl, tuned.get("learned", base_tuning))
+        out["c5"] = bench_c5(cb, gen, ctl, log)
     if not args.no_cpu_baseline:
         log("[bench] timing the CPU baseline")
         out["cpu_baseline"] = cpu_baseline(oracle, A)
