@@ -283,12 +283,13 @@ __device__ __forceinline__ void staged_copy_out(const uint32_t *src, char *dst, 
 }
 
 // Writes the set bits of 256 bitmap words as columns at out[pos ...] (never
-// at or past lim): lane l holds words 4l .. 4l + 3 (x[0..3]; bit b of x[e]
-// is column cb + 32 * (4l + e) + b).  One wave scan per round; a round whose
-// columns fit the staging area goes through it, a denser one is written two
-// words at a time with a lane per bit (64 consecutive columns per ballot,
-// stores in order).  Returns the round's set bits.
-__device__ __forceinline__ uint32_t emit_round4_staged(const uint32_t (&x)[4], uint32_t cb, uint64_t pos,
+// at or past lim): lane l holds four words x[0..3], bit b of x[e] being
+// column cw[e] + b; the words are in column order by (lane, e).  One wave
+// scan per round; a round whose columns fit the staging area goes through
+// it, a denser one is written two words at a time with a lane per bit (64
+// consecutive columns per ballot, stores in order).  Returns the round's set
+// bits.
+__device__ __forceinline__ uint32_t emit_round4_staged(const uint32_t (&x)[4], const uint32_t (&cw)[4], uint64_t pos,
                                                        uint32_t *out, uint64_t lim, uint32_t *sg) {
     const uint32_t l = (uint32_t)lane();
     const uint32_t pc = (uint32_t)(__popc(x[0]) + __popc(x[1]) + __popc(x[2]) + __popc(x[3]));
@@ -306,9 +307,8 @@ __device__ __forceinline__ uint32_t emit_round4_staged(const uint32_t (&x)[4], u
 #pragma unroll
         for (uint32_t e = 0; e < 4; ++e) {
             uint32_t w = x[e];
-            const uint32_t c0 = cb + 32u * (4u * l + e);
             while (w) {
-                sg[padded ? q + (q >> 5) : q] = c0 + (uint32_t)__builtin_ctz(w);
+                sg[padded ? q + (q >> 5) : q] = cw[e] + (uint32_t)__builtin_ctz(w);
                 ++q;
                 w &= w - 1u;
             }
@@ -318,8 +318,8 @@ __device__ __forceinline__ uint32_t emit_round4_staged(const uint32_t (&x)[4], u
         else staged_copy_out<64>(sg + l, dst, l, n);
         __builtin_amdgcn_wave_barrier();
     } else {
-        // words 4j + 2h and 4j + 2h + 1 of lane j: lanes 0..31 take the
-        // first word's bits, lanes 32..63 the second's
+        // words (j, 2h) and (j, 2h + 1): lanes 0..31 take the first word's
+        // bits, lanes 32..63 the second's
         const uint32_t sh = l & 31u;
         uint32_t p = 0;
         for (uint32_t j = 0; j < WAVE; ++j) {
@@ -328,11 +328,13 @@ __device__ __forceinline__ uint32_t emit_round4_staged(const uint32_t (&x)[4], u
                 const uint32_t w0 = (uint32_t)__builtin_amdgcn_readlane((int)x[2 * h], (int)j);
                 const uint32_t w1 = (uint32_t)__builtin_amdgcn_readlane((int)x[2 * h + 1], (int)j);
                 if ((w0 | w1) == 0) continue;
+                const uint32_t b0 = (uint32_t)__builtin_amdgcn_readlane((int)cw[2 * h], (int)j);
+                const uint32_t b1 = (uint32_t)__builtin_amdgcn_readlane((int)cw[2 * h + 1], (int)j);
                 const uint32_t v = l < 32 ? w0 : w1;
                 const bool set = (v >> sh) & 1u;
                 const uint64_t m = __ballot(set);
                 const uint32_t at = p + lanes_below(m);
-                if (set && at < n) *reinterpret_cast<uint32_t *>(dst + 4u * at) = cb + 32u * (4u * j + 2u * h) + l;
+                if (set && at < n) *reinterpret_cast<uint32_t *>(dst + 4u * at) = (l < 32 ? b0 : b1) + sh;
                 p += (uint32_t)__popcll(m);
             }
         }

@@ -132,23 +132,25 @@ __global__ void k_set_positions(const u32 *__restrict__ nnz_row, const u64 *__re
     c_rows[q] = a_rows[row0 + i];
 }
 
-// Expands one dumped bitmap row (one workgroup per row) into its columns at
-// their final positions.  Each of the 2 waves owns a contiguous half of
-// the row's words; its start position is the prefix of the chunk counts
-// before its first chunk, so the waves need no exchange.  A wave loads its
-// words 1024 at a time (four rounds of 256, 16 bytes per lane per round),
+// Writes one dumped row (one workgroup per row) into its columns at their
+// final positions.  Each of the 2 waves owns a contiguous half of the row's
+// words; its start position is the prefix of the chunk counts before its
+// first chunk, so the waves need no exchange.  A bitmap slice is loaded 1024
+// words at a time (four rounds of 256, 16 bytes per lane per round),
 // double-buffered (the next 1024 words' loads are in flight while the
-// current ones are expanded), and writes them through its own LDS staging
-// area (dev::emit_round4_staged: one wave scan per round), so every store
-// instruction is one contiguous run of columns.  Two-wave workgroups (8.4 KB of staging
-// per wave) let nine rows share a CU, so one row's load latency overlaps
-// another's expansion (at 256 threads, four rows: 11% slower).
-// gpre = exclusive prefix of chunk_cnt over all slots.
+// current ones are expanded); a slice dumped as (word index, word) pairs is
+// read 256 pairs per round.  Either way the words go through the wave's LDS
+// staging area (dev::emit_round4_staged: one wave scan per round), so every
+// store instruction is one contiguous run of columns.  Two-wave workgroups
+// (8.4 KB of staging per wave) let nine rows share a CU, so one row's load
+// latency overlaps another's expansion (at 256 threads, four rows: 11%
+// slower).  gpre = exclusive prefix of chunk_cnt over all slots.
 constexpr int EMIT_WG = 128;
 __global__ __launch_bounds__(EMIT_WG) void k_bitmap_emit(const uint4 *__restrict__ store, u32 nch,
                                                          const u64 *__restrict__ gpre, const u32 *__restrict__ list,
                                                          const u32 *__restrict__ perm, const u64 *__restrict__ off,
-                                                         u32 words_per_row, u32 *__restrict__ cols) {
+                                                         u32 words_per_row, u32 slice_words,
+                                                         const u32 *__restrict__ fmt, u32 *__restrict__ cols) {
     constexpr u32 NW = EMIT_WG / 64, ROUNDS = 4, SPAN = ROUNDS * 256;
     __shared__ u32 stage[NW][dev::STAGE_WORDS];
     const u32 slot = blockIdx.x;  // slots are claimed densely from 0
@@ -156,18 +158,19 @@ __global__ __launch_bounds__(EMIT_WG) void k_bitmap_emit(const uint4 *__restrict
     u32 *out = cols + off[r];
     const u64 lim = off[r + 1] - off[r];
     const u32 wv = dev::wave_id(), l = (u32)dev::lane();
-    // words_per_row is a multiple of 2^14, so wpw is a multiple of 2 * SPAN
-    // and of EMIT_CHUNK
+    // words_per_row and slice_words are multiples of 2^14, so wpw and every
+    // slice boundary inside it are multiples of 2 * SPAN and of EMIT_CHUNK
     const u32 wpw = words_per_row / NW;
-    const u32 first = wv * wpw;
-    const u32 *words = reinterpret_cast<const u32 *>(store) + (u64)slot * words_per_row + first;
+    const u32 first = wv * wpw, last = first + wpw;
+    const u32 nsl = words_per_row / slice_words;
+    const u32 *row = reinterpret_cast<const u32 *>(store) + (u64)slot * words_per_row;
     u64 p = gpre[(u64)slot * nch + first / EMIT_CHUNK] - gpre[(u64)slot * nch];
-    // a batch is ROUNDS rounds of 256 words, lane l holding words 4l .. 4l + 3
-    // of each (16-byte loads)
+    // a bitmap batch is ROUNDS rounds of 256 words, lane l holding words
+    // 4l .. 4l + 3 of each (16-byte loads); s0 counts words from the row's start
     auto load = [&](u32(&w)[ROUNDS][4], u32 s0) {
 #pragma unroll
         for (u32 k = 0; k < ROUNDS; ++k) {
-            const uint4 v = *reinterpret_cast<const uint4 *>(words + s0 + k * 256 + 4 * l);
+            const uint4 v = *reinterpret_cast<const uint4 *>(row + s0 + k * 256 + 4 * l);
             w[k][0] = v.x;
             w[k][1] = v.y;
             w[k][2] = v.z;
@@ -176,16 +179,59 @@ __global__ __launch_bounds__(EMIT_WG) void k_bitmap_emit(const uint4 *__restrict
     };
     auto expand = [&](const u32(&w)[ROUNDS][4], u32 s0) {
 #pragma unroll
-        for (u32 k = 0; k < ROUNDS; ++k)
-            p += dev::emit_round4_staged(w[k], (first + s0 + k * 256) * 32u, p, out, lim, stage[wv]);
+        for (u32 k = 0; k < ROUNDS; ++k) {
+            const u32 c0 = (s0 + k * 256 + 4 * l) * 32u;
+            const u32 cw[4] = {c0, c0 + 32u, c0 + 64u, c0 + 96u};
+            p += dev::emit_round4_staged(w[k], cw, p, out, lim, stage[wv]);
+        }
     };
-    u32 wa[ROUNDS][4], wb[ROUNDS][4];
-    load(wa, 0);
-    for (u32 s0 = 0; s0 < wpw; s0 += 2 * SPAN) {
-        load(wb, s0 + SPAN);
-        expand(wa, s0);
-        if (s0 + 2 * SPAN < wpw) load(wa, s0 + 2 * SPAN);
-        expand(wb, s0 + SPAN);
+    // the wave's words one slice at a time
+    for (u32 seg = first; seg < last;) {
+        const u32 sidx = seg / slice_words, s0w = sidx * slice_words;
+        const u32 send = s0w + slice_words < last ? s0w + slice_words : last;
+        const u32 f = fmt[(u64)slot * nsl + sidx];
+        if (f) {
+            // the pairs whose (slice-relative) word index is in [seg, send)
+            const uint2 *pairs = reinterpret_cast<const uint2 *>(row + (u64)s0w);
+            const u32 npairs = f - 1u;
+            auto lower = [&](u32 key) {
+                u32 lo = 0, hi = npairs;
+                while (lo < hi) {
+                    const u32 mid = (lo + hi) / 2;
+                    if (pairs[mid].x < key) lo = mid + 1;
+                    else hi = mid;
+                }
+                return lo;
+            };
+            const u32 pa = seg == s0w ? 0u : lower(seg - s0w);
+            const u32 pb = send == s0w + slice_words ? npairs : lower(send - s0w);
+            for (u32 k0 = pa; k0 < pb; k0 += 256) {
+                u32 x[4], cw[4];
+#pragma unroll
+                for (u32 e = 0; e < 4; ++e) {
+                    const u32 k = k0 + 4 * l + e;
+                    if (k < pb) {
+                        const uint2 pr = pairs[k];
+                        x[e] = pr.y;
+                        cw[e] = (s0w + pr.x) * 32u;
+                    } else {
+                        x[e] = 0;
+                        cw[e] = 0;
+                    }
+                }
+                p += dev::emit_round4_staged(x, cw, p, out, lim, stage[wv]);
+            }
+        } else {
+            u32 wa[ROUNDS][4], wb[ROUNDS][4];
+            load(wa, seg);
+            for (u32 s0 = seg; s0 < send; s0 += 2 * SPAN) {
+                load(wb, s0 + SPAN);
+                expand(wa, s0);
+                if (s0 + 2 * SPAN < send) load(wa, s0 + 2 * SPAN);
+                expand(wb, s0 + SPAN);
+            }
+        }
+        seg = send;
     }
 }
 
@@ -214,6 +260,7 @@ void setup_dump(clb_controls_s *ctl, const Setup &s, const BinHost &h, DumpBufs 
     // a dumped row keeps every slice: nslices windows of 2^bm_bits words
     if (d.nbm == 0 || (nslices << s.cfg.bm_bits) > (1ull << 26)) return;
     d.words = (u32)(nslices << s.cfg.bm_bits);
+    d.slice_words = 1u << s.cfg.bm_bits;
     d.nch = d.words / EMIT_CHUNK;
     const u32 dense_min = ((1u << s.cfg.bm_bits) / 1024u) * knob;
     // slots for the rows likely to be dumped: those whose size key reaches a
@@ -249,11 +296,12 @@ void setup_dump(clb_controls_s *ctl, const Setup &s, const BinHost &h, DumpBufs 
     d.list.reset(ctl, slots);
     d.count.reset(ctl, 1);
     d.isdense.reset(ctl, s.nrows);
+    d.fmt.reset(ctl, (u64)slots * nslices);  // written for every slice of a claimed slot
     fill_u32(ctl, d.cnt.ptr, d.cnt.n, 0);
     fill_u32(ctl, d.count.ptr, 1, 0);
     fill_u32(ctl, d.isdense.ptr, s.nrows, 0);
     // the decision is taken on slice 0's count
-    o.dump = Dump{d.store.ptr, d.cnt.ptr, d.list.ptr, d.count.ptr, d.isdense.ptr, dense_min, slots};
+    o.dump = Dump{d.store.ptr, d.cnt.ptr, d.list.ptr, d.count.ptr, d.isdense.ptr, dense_min, slots, d.fmt.ptr};
     d.on = true;
 }
 
@@ -280,7 +328,7 @@ void emit_launch(hipStream_t es, u32 count, const u32 *perm, const BinHost &h, c
     if (!count) return;
     if (lt) lt->begin(es);
     k_bitmap_emit<<<count, EMIT_WG, 0, es>>>(d.store.ptr, d.nch, gpre, d.list.ptr, perm + h.off[BIN_BITMAP0], off,
-                                            d.words, cols);
+                                            d.words, d.slice_words, d.fmt.ptr, cols);
     CLB_LAUNCHED("bitmap_emit");
     if (lt) lt->end((int)CLB_KF(pass, CLB_KK_EMIT), es);
 }

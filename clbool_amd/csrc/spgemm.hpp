@@ -68,9 +68,9 @@ struct BinHost {
 // Scratch of the dense-bitmap path (see Dump in spgemm_kernels.hpp).
 struct DumpBufs {
     bool on = false;
-    u32 nbm = 0, nch = 0, words = 0, slots = 0;
+    u32 nbm = 0, nch = 0, words = 0, slots = 0, slice_words = 0;
     DevArray<uint4> store;
-    DevArray<u32> cnt, list, count, isdense;
+    DevArray<u32> cnt, list, count, isdense, fmt;
 };
 // Allocates the dense-bitmap scratch for the bitmap bin of `h` (nothing when
 // the window is below 2^14 words, the knob disables it or it would not fit)

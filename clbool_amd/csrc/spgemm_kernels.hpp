@@ -443,10 +443,12 @@ __device__ __forceinline__ u32 hash_insert(u32 *table, u32 col, u32 max_probe) {
 constexpr int SYM = 0, NUM = 1, ONE = 2;
 constexpr u32 FAILED = 0xFFFFFFFFu;
 
-// Dense bitmap rows (single-slice windows of 2^14 or 2^15 words) leave the
-// gather kernel as a bitmap in global memory plus popcounts per 1024-word
-// chunk; k_bitmap_emit expands them later into their final positions with
-// coalesced stores.  store == nullptr disables this path.
+// Dense bitmap rows (windows of 2^14 or 2^15 words, every slice of the row)
+// leave the gather kernel in global memory, each slice as its bitmap or, when
+// at most a quarter of its words are nonzero, as (word index, word) pairs of
+// those words, plus popcounts per 1024-word chunk; k_bitmap_emit expands them
+// later into their final positions with coalesced stores.  store == nullptr
+// disables this path.
 constexpr u32 EMIT_CHUNK = 1024;  // words per emission chunk
 // Slots are handed out in the order rows decide to dump (*count counts the
 // claims); a row that finds all `capacity` slots taken is not dumped.
@@ -458,6 +460,8 @@ struct Dump {
     u32 *isdense;      // [row] 1 if the row was dumped
     u32 dense_min;     // dump rows with at least this many columns in their first window
     u32 capacity;      // slots in store
+    u32 *fmt;          // [slot][nslices]: 0 = the slice's bitmap, else 1 + the number of
+                       // (word index, word) pairs of its nonzero words
 };
 
 struct Out {
@@ -792,24 +796,33 @@ __device__ __forceinline__ void bitmap_row(const Mats &g, const u32 *__restrict_
                 }
             };
             u32 c[R];
-            u32 cnt = 0;
+            u32 cnt = 0, nzw = 0;  // set bits and nonzero words of this lane
 #pragma unroll
             for (u32 m = 0; m < R; ++m) {
                 u32 x[E];
                 words(m, x);
                 c[m] = 0;
 #pragma unroll
-                for (u32 e = 0; e < E; ++e) c[m] += __popc(x[e]);
+                for (u32 e = 0; e < E; ++e) {
+                    c[m] += __popc(x[e]);
+                    nzw += x[e] != 0u;
+                }
                 cnt += c[m];
             }
-            const u32 wcnt = dev::wave_sum(cnt);
+            // one exchange carries both wave totals: set bits (at most
+            // 32 * WPW <= 2^16) above 12 bits of nonzero words (at most WPW
+            // <= 2^11)
+            static_assert(WPW <= 2048, "packed wave totals");
+            const u32 wcnt = dev::wave_sum((cnt << 12) | nzw);
             if (l == 0) red[wv] = wcnt;
             __syncthreads();
-            u32 woff = 0, tot = 0;
+            u32 woff = 0, tot = 0, nzoff = 0, nztot = 0;
             for (u32 v = 0; v < NW; ++v) {
-                const u32 t = red[v];
+                const u32 t = red[v] >> 12, z = red[v] & 0xFFFu;
                 woff += v < wv ? t : 0u;
                 tot += t;
+                nzoff += v < wv ? z : 0u;
+                nztot += z;
             }
             __syncthreads();
             if constexpr (WBITS >= 14) if (o.dump.store) {
@@ -842,10 +855,42 @@ __device__ __forceinline__ void bitmap_row(const Mats &g, const u32 *__restrict_
                         cc = dev::wave_sum(cc);
                         if (l == 0) o.dump.chunk_cnt[sl * NCH + wv * CPW + k] = cc;
                     }
-                    uint4 *dst = o.dump.store + sl * (W / 4) + wv * (WPW / 4);
-                    const uint4 *src4 = reinterpret_cast<const uint4 *>(wb);
+                    // a slice with at most W / 4 nonzero words is dumped as
+                    // (word index, word) pairs of those words (at most half
+                    // the bitmap's bytes): one wave scan per round, at most
+                    // four 8-byte stores per lane
+                    const bool as_pairs = nztot <= W / 4;  // uniform
+                    if (as_pairs) {
+                        uint2 *pairs = reinterpret_cast<uint2 *>(o.dump.store + sl * (W / 4));
+                        u32 q0 = nzoff;
+#pragma unroll 1
+                        for (u32 m = 0; m < R; ++m) {
+                            u32 x[E];
+                            u32 nz = 0;
+                            if (c[m]) {
+                                words(m, x);
+#pragma unroll
+                                for (u32 e = 0; e < E; ++e) nz += x[e] != 0u;
+                            }
+                            const u32 inc = dev::wave_incl_sum_dpp(nz);
+                            const u32 rt = (u32)__builtin_amdgcn_readlane((int)inc, 63);
+                            if (rt == 0) continue;  // uniform
+                            if (nz) {
+                                u32 q = q0 + (inc - nz);
+                                const u32 i0 = wv * WPW + (m * dev::WAVE + (u32)l) * E;
+#pragma unroll
+                                for (u32 e = 0; e < E; ++e)
+                                    if (x[e]) pairs[q++] = make_uint2(i0 + e, x[e]);
+                            }
+                            q0 += rt;
+                        }
+                    } else {
+                        uint4 *dst = o.dump.store + sl * (W / 4) + wv * (WPW / 4);
+                        const uint4 *src4 = reinterpret_cast<const uint4 *>(wb);
 #pragma unroll 4
-                    for (u32 m = 0; m < R; ++m) dst[m * dev::WAVE + (u32)l] = src4[m * dev::WAVE + (u32)l];
+                        for (u32 m = 0; m < R; ++m) dst[m * dev::WAVE + (u32)l] = src4[m * dev::WAVE + (u32)l];
+                    }
+                    if (threadIdx.x == 0) o.dump.fmt[sl] = as_pairs ? nztot + 1u : 0u;
                     written += tot;
                     __syncthreads();  // before the next slice clears the bitmap
                     continue;
