@@ -445,7 +445,7 @@ constexpr u32 FAILED = 0xFFFFFFFFu;
 
 // Dense bitmap rows (windows of 2^14 or 2^15 words, every slice of the row)
 // leave the gather kernel in global memory, each slice as its bitmap or, when
-// at most a quarter of its words are nonzero, as (word index, word) pairs of
+// at most a third of its words are nonzero, as (word index, word) pairs of
 // those words, plus popcounts per 1024-word chunk; k_bitmap_emit expands them
 // later into their final positions with coalesced stores.  store == nullptr
 // disables this path.
@@ -855,11 +855,12 @@ __device__ __forceinline__ void bitmap_row(const Mats &g, const u32 *__restrict_
                         cc = dev::wave_sum(cc);
                         if (l == 0) o.dump.chunk_cnt[sl * NCH + wv * CPW + k] = cc;
                     }
-                    // a slice with at most W / 4 nonzero words is dumped as
-                    // (word index, word) pairs of those words (at most half
-                    // the bitmap's bytes): one wave scan per round, at most
-                    // four 8-byte stores per lane
-                    const bool as_pairs = nztot <= W / 4;  // uniform
+                    // a slice with at most W / 3 nonzero words is dumped as
+                    // (word index, word) pairs of those words (at most two
+                    // thirds of the bitmap's bytes; W / 4 and W / 8 measured
+                    // slower): one wave scan per round, at most four 8-byte
+                    // stores per lane
+                    const bool as_pairs = nztot <= W / 3;  // uniform
                     if (as_pairs) {
                         uint2 *pairs = reinterpret_cast<uint2 *>(o.dump.store + sl * (W / 4));
                         u32 q0 = nzoff;
