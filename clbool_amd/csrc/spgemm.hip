@@ -379,9 +379,10 @@ void ensure_splits(clb_controls_s *ctl, Setup &s, const BinHost &h) {
     const u64 span = 32ull << s.cfg.bm_bits;
     const u64 nsl = ((u64)b.ncols + span - 1) / span;
     if (nsl <= 1 || !b.nzr || (u64)s.mapn * (nsl + 1) >= (1ull << 32)) return;
-    // written for stored B rows only: the gather reads a row's entries only
-    // when its range in brange is nonempty
+    // zero for the rows B does not store (an empty part in every slice), so
+    // the gather reads it without looking the row up in brange
     s.splits.reset(ctl, (u64)s.mapn * (nsl + 1));
+    fill_u32(ctl, s.splits.ptr, s.splits.n, 0u);
     k_slice_splits<<<div_up(b.nzr, 256), 256, 0, ctl->stream>>>(u32p(b.rpt), u32p(b.rows), u32p(b.cols), b.nzr,
                                                                  s.mapn, (u32)nsl, span, s.splits.ptr);
     CLB_LAUNCHED("slice_splits");

@@ -54,8 +54,9 @@ struct Mats {
     // Column slices of the bitmap kernels (nsl > 1 when ncols exceeds one
     // window of 2^split_bits words): splits[k * (nsl + 1) + s] is the
     // position in b_cols of B row k's first column in slice s (s = nsl: the
-    // row's end), so a slice reads only its own part of each B row.  A kernel
-    // with a window 2^d times as wide takes every 2^d-th boundary.
+    // row's end), so a slice reads only its own part of each B row; all
+    // entries of a row B does not store are zero.  A kernel with a window
+    // 2^d times as wide takes every 2^d-th boundary.
     const u32 *splits;
     u32 nsl;
     u32 split_bits;
@@ -73,6 +74,32 @@ __device__ __forceinline__ uint2 b_range(const Mats &g, u32 k) {
     return g.brange[k];
 }
 constexpr u32 NOSLICE = 0xFFFFFFFFu;
+
+// {first position, length} in b_cols of what the A nonzero with column k
+// reads: the whole B row (slice == NOSLICE) or its part in fine slices
+// [slice * sstep, (slice + 1) * sstep) through the split table, which is
+// zero for rows B does not store, so that path needs no brange lookup.
+// k == EMPTY gives {0, 0}.
+__device__ __forceinline__ uint2 b_part(const Mats &g, u32 k, u32 slice, u32 sstep) {
+    if (slice == NOSLICE) {
+        const uint2 rr = b_range(g, k);
+        return make_uint2(rr.x, rr.y - rr.x);
+    }
+    if (k >= g.mapn) return make_uint2(0, 0);
+    const u32 *sp = g.splits + (u64)k * (g.nsl + 1);
+    const u32 f0 = slice * sstep, f1 = f0 + sstep < g.nsl ? f0 + sstep : g.nsl;
+    const u32 b0 = sp[f0];
+    return make_uint2(b0, sp[f1] - b0);
+}
+
+// Lookahead over the slices of one row: k0 is this lane's A column in the
+// row's first chunk (EMPTY past the row's end) and part its B part in the
+// slice being gathered; the gather of a slice leaves the next slice's part
+// in it.
+struct RowAhead {
+    u32 k0;
+    uint2 part;
+};
 
 // Bitmap emission.  Each wave owns a contiguous range of the bitmap's words
 // and walks it 64 words at a time (dev::emit_step): a step whose 64 words
@@ -147,28 +174,41 @@ __device__ __forceinline__ u32 classify(u32 P, u32 key, const BinCfg &c) {
 constexpr int UNROLL = 4;
 __host__ __device__ constexpr int BIG_WORDS(int WG) { return 3 * WG + 2 * (WG / 64 + 2); }
 
-template <int WG, int UB, u32 BIGMIN, typename F, typename FF>
+template <int WG, int UB, u32 BIGMIN, bool AHEAD = false, typename F, typename FF>
 __device__ __forceinline__ void for_each_product(const Mats &g, u32 js, u32 je, u32 *big, F &&f, FF &&ff,
-                                                 u32 slice = NOSLICE, u32 sstep = 1) {
+                                                 u32 slice = NOSLICE, u32 sstep = 1, RowAhead *ah = nullptr,
+                                                 u32 next_slice = NOSLICE) {
     const int l = dev::lane();
     constexpr int NW = WG / dev::WAVE;
     constexpr u32 WIN = UB * 64;
+    // AHEAD (a metadata pipeline, for kernels whose occupancy LDS limits:
+    // it costs ~7 VGPRs): `part` holds the B part of this lane's nonzero in
+    // the current chunk and `knext` the A column of its nonzero in the next
+    // chunk.  Before its gather, a chunk issues the loads of the next chunk's
+    // part and of the column two chunks ahead (the first chunk, given a
+    // RowAhead, also the next slice's first part), so that their latency
+    // overlaps the gather's loads instead of preceding them.
+    auto col_at = [&](u64 j) { return j < je ? g.a_cols[j] : EMPTY; };
+    uint2 part = make_uint2(0u, 0u);
+    u32 knext = EMPTY;
+    if constexpr (AHEAD) {
+        part = ah ? ah->part : b_part(g, col_at((u64)js + threadIdx.x), slice, sstep);
+        knext = col_at((u64)js + WG + threadIdx.x);
+    }
     for (u32 base = js; base < je; base += WG) {
-        const u32 j = base + threadIdx.x;
-        u32 bx = 0, len = 0;
-        if (j < je) {
-            const u32 k = g.a_cols[j];
-            const uint2 rr = b_range(g, k);
-            if (slice == NOSLICE) {
-                bx = rr.x;
-                len = rr.y - rr.x;
-            } else if (rr.y > rr.x) {  // split entries exist for stored B rows only
-                // fine slices [slice * sstep, (slice + 1) * sstep) of the row
-                const u32 *sp = g.splits + (u64)k * (g.nsl + 1);
-                const u32 f0 = slice * sstep, f1 = f0 + sstep < g.nsl ? f0 + sstep : g.nsl;
-                bx = sp[f0];
-                len = sp[f1] - bx;
+        u32 bx, len;
+        if constexpr (AHEAD) {
+            bx = part.x;
+            len = part.y;
+            if (ah && base == js && next_slice != NOSLICE) ah->part = b_part(g, ah->k0, next_slice, sstep);
+            if ((u64)base + WG < je) {
+                part = b_part(g, knext, slice, sstep);
+                knext = col_at((u64)base + 2 * WG + threadIdx.x);
             }
+        } else {
+            const uint2 p = b_part(g, col_at((u64)base + threadIdx.x), slice, sstep);
+            bx = p.x;
+            len = p.y;
         }
         // list entry e: big[3e] = first B position, big[3e+1] = end,
         // big[3e+2] = windows before it.  Window counts of distinct B rows
@@ -311,10 +351,11 @@ __device__ __forceinline__ void for_each_product(const Mats &g, u32 js, u32 je, 
 }
 
 // The same with one callable for every product.
-template <int WG, int UB, u32 BIGMIN, typename F>
+template <int WG, int UB, u32 BIGMIN, bool AHEAD = false, typename F>
 __device__ __forceinline__ void for_each_product(const Mats &g, u32 js, u32 je, u32 *big, F &&f,
-                                                 u32 slice = NOSLICE, u32 sstep = 1) {
-    for_each_product<WG, UB, BIGMIN>(g, js, je, big, f, f, slice, sstep);
+                                                 u32 slice = NOSLICE, u32 sstep = 1, RowAhead *ah = nullptr,
+                                                 u32 next_slice = NOSLICE) {
+    for_each_product<WG, UB, BIGMIN, AHEAD>(g, js, je, big, f, f, slice, sstep, ah, next_slice);
 }
 
 // ---------------------------------------------------------------------------
@@ -726,7 +767,13 @@ __device__ __forceinline__ void bitmap_row(const Mats &g, const u32 *__restrict_
     u32 slot = 0;          // its slot in the dump
     u32 *out = MODE == SYM ? nullptr : o.cols + o.off[r];
     const u64 lim = row_limit<MODE>(o, r);
+    RowAhead ah{EMPTY, make_uint2(0u, 0u)};
+    if (use_splits && nslices > 1) {
+        if (threadIdx.x < je - js) ah.k0 = g.a_cols[js + threadIdx.x];
+        ah.part = b_part(g, ah.k0, 0, sstep);
+    }
     for (u32 s = 0; s < nslices; ++s) {
+        const u32 sn = s + 1 < nslices ? s + 1 : NOSLICE;
         const u64 base = (u64)s * SPAN;
         {
             uint4 *bm4 = reinterpret_cast<uint4 *>(bm);
@@ -745,16 +792,16 @@ __device__ __forceinline__ void bitmap_row(const Mats &g, const u32 *__restrict_
         auto setdw = [&](u32 col) { dev::wave_or_sorted_bits(bm, (u32)((u64)col - base)); };
         constexpr int UBW = MODE == SYM ? 16 : 8;
         if (nslices == 1) {
-            if constexpr (COMB) for_each_product<WG, UBW, LONG_ROW>(g, js, je, big, set1, setw);
-            else for_each_product<WG, UBW, LONG_ROW>(g, js, je, big, set1);
+            if constexpr (COMB) for_each_product<WG, UBW, LONG_ROW, true>(g, js, je, big, set1, setw);
+            else for_each_product<WG, UBW, LONG_ROW, true>(g, js, je, big, set1);
         } else {
             // only the slice's part of each B row is read (with the split table)
             if constexpr (COMB) {
-                if (use_splits) for_each_product<WG, UBW, LONG_ROW>(g, js, je, big, setu, setdw, s, sstep);
-                else for_each_product<WG, UBW, LONG_ROW>(g, js, je, big, setd, NOSLICE, sstep);
+                if (use_splits) for_each_product<WG, UBW, LONG_ROW, true>(g, js, je, big, setu, setdw, s, sstep, &ah, sn);
+                else for_each_product<WG, UBW, LONG_ROW, true>(g, js, je, big, setd, NOSLICE, sstep);
             } else {
-                if (use_splits) for_each_product<WG, UBW, LONG_ROW>(g, js, je, big, setu, s, sstep);
-                else for_each_product<WG, UBW, LONG_ROW>(g, js, je, big, setd, NOSLICE, sstep);
+                if (use_splits) for_each_product<WG, UBW, LONG_ROW, true>(g, js, je, big, setu, s, sstep, &ah, sn);
+                else for_each_product<WG, UBW, LONG_ROW, true>(g, js, je, big, setd, NOSLICE, sstep);
             }
         }
         __syncthreads();
