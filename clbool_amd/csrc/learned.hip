@@ -514,9 +514,23 @@ struct OnePass {
     clb_dcsr out{};
     clb_spgemm_stats st{};
     hipEvent_t joined = nullptr, done = nullptr;
+    // the end of the block's row kernels on each side stream (launch_bins
+    // with a deferred join); post makes the controls' stream wait for them
+    hipEvent_t bins[clb_controls_s::NSIDE] = {};
     bool tail = false, timing = false, replan = false;
+    // kept from launch for post
+    clb_dcsr a{}, b{};
+    PlanRows prc{};
+    bool planned = false, posted = false, empty = false;
+    u32 *cap = nullptr;
+    const float *pred = nullptr;
     std::chrono::steady_clock::time_point t0;
     ~OnePass() {
+        for (hipEvent_t &e : bins)
+            if (e) {
+                (void)hipEventSynchronize(e);
+                (void)hipEventDestroy(e);
+            }
         if (tail)
             for (int k = 0; k < clb_controls_s::NTAIL; ++k) (void)hipStreamSynchronize(ctl->tail[k]);
         if (joined) (void)hipEventDestroy(joined);
@@ -527,8 +541,8 @@ struct OnePass {
 
 void OnePassDeleter::operator()(OnePass *p) const { delete p; }
 
-OnePassPtr one_pass_start(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &b, u32 rb, u32 re, bool timing,
-                          bool bound_only, const PlanRows *pr) {
+OnePassPtr one_pass_launch(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &b, u32 rb, u32 re, bool timing,
+                           bool bound_only, const PlanRows *pr) {
     OnePassPtr P(new OnePass());
     OnePass &p = *P;
     p.ctl = ctl;
@@ -537,6 +551,12 @@ OnePassPtr one_pass_start(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr
     p.t0 = std::chrono::steady_clock::now();
     p.out.nrows = a.nrows;
     p.out.ncols = b.ncols;
+    p.a = a;
+    p.b = b;
+    if (pr) {
+        p.prc = *pr;
+        p.planned = true;
+    }
     clb_spgemm_stats &st = p.st;
     Setup &s = p.s;
     const u32 nrows = re - rb;
@@ -571,13 +591,36 @@ OnePassPtr one_pass_start(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr
     st.rows = nrows;
     st.n_bins = NBINS;
     st.capacity = sc[4];
-    if (st.products == 0) return P;
+    p.cap = cap;
+    p.pred = pred;
+    if (st.products == 0) {
+        p.empty = true;
+        return P;
+    }
     p.temp.reset(ctl, sc[4] ? sc[4] : 1);
     u32 *fails = reinterpret_cast<u32 *>(s.scal.ptr + 5);
     Out o{p.nnz_row.ptr, p.toff.ptr, p.temp.ptr, cap, fails, Dump{}, ONE_PASS_PROBES};
     setup_dump(ctl, s, p.h, p.dump, o, true, p.perm.ptr, cap);
     ensure_splits(ctl, s, p.h);
-    launch_bins(ctl, s, p.perm.ptr, p.h.off, ONE, o, &p.lt, CLB_KP_ONEPASS);
+    for (hipEvent_t &e : p.bins) CLB_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming), EVENT_WAITING_ERROR);
+    launch_bins(ctl, s, p.perm.ptr, p.h.off, ONE, o, &p.lt, CLB_KP_ONEPASS, p.bins);
+    return P;
+}
+
+void one_pass_post(OnePass &p) {
+    if (p.posted) return;
+    p.posted = true;
+    if (p.empty) return;
+    clb_controls_s *ctl = p.ctl;
+    clb_spgemm_stats &st = p.st;
+    Setup &s = p.s;
+    const clb_dcsr &a = p.a, &b = p.b;
+    const PlanRows *pr = p.planned ? &p.prc : nullptr;
+    const u32 nrows = s.nrows;
+    const bool timing = p.timing;
+    const float *pred = p.pred;
+    join_deferred(ctl, p.bins);
+    u32 *fails = reinterpret_cast<u32 *>(s.scal.ptr + 5);
     u32 nfail = 0;
     readback(ctl, &nfail, fails, 4);
     st.fallback_rows = nfail;
@@ -611,7 +654,9 @@ OnePassPtr one_pass_start(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr
         u32 *fails2 = reinterpret_cast<u32 *>(s.scal.ptr + 7);
         Out o2{p.nnz_row.ptr, p.soff.ptr, p.side.ptr, p.key2.ptr, fails2, Dump{}, 0};  // exact: full probing
         ensure_splits(ctl, s, h2);
-        launch_bins(ctl, s, p.perm2.ptr, h2.off, ONE, o2, &p.lt, CLB_KP_FALLBACK);
+        // on the controls' stream: the side streams may already hold the
+        // next block's row kernels, which this block's re-run must not wait for
+        launch_bins(ctl, s, p.perm2.ptr, h2.off, ONE, o2, &p.lt, CLB_KP_FALLBACK, nullptr, true);
         s.cfg = keep;
         u32 nfail2 = 0;
         readback(ctl, &nfail2, fails2, 4);
@@ -629,15 +674,15 @@ OnePassPtr one_pass_start(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr
         CLB_HIP(hipMemcpyAsync(pr->size, p.nnz_row.ptr, 4ull * nrows, hipMemcpyDeviceToDevice, ctl->stream),
                 COPY_BUFFER_ERROR);
         sync(ctl);
-        *pr->replan = true;
+        if (pr->replan) *pr->replan = true;
         p.replan = true;
-        return P;
+        return;
     }
     if (nnz > 0xFFFFFFFFull)
         raise(INVALID_VALUE, "result has more nonzeros than uint32 indices allow; multiply by row blocks");
     st.nnz = nnz;
     st.nzr = nzr;
-    if (nnz == 0) return P;
+    if (nnz == 0) return;
     p.err.reset(ctl, 1);
     fill_u64(ctl, reinterpret_cast<u64 *>(p.err.ptr), 1, 0ull);
     clb_buffer_s *cols = buffer_new(ctl, 4ull * nnz);
@@ -663,7 +708,13 @@ OnePassPtr one_pass_start(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr
     CLB_HIP(hipEventRecord(p.joined, t0s), EVENT_WAITING_ERROR);
     CLB_HIP(hipStreamWaitEvent(t1s, p.joined, 0), EVENT_WAITING_ERROR);
     CLB_HIP(hipEventRecord(p.done, t1s), EVENT_WAITING_ERROR);
-    return P;
+}
+
+OnePassPtr one_pass_start(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &b, u32 rb, u32 re, bool timing,
+                          bool bound_only, const PlanRows *pr) {
+    OnePassPtr p = one_pass_launch(ctl, a, b, rb, re, timing, bound_only, pr);
+    one_pass_post(*p);
+    return p;
 }
 
 bool one_pass_replan(const OnePass &p) { return p.replan; }
@@ -677,6 +728,7 @@ u64 one_pass_bytes(const OnePass &p) {
 
 void one_pass_finish(OnePass &p, clb_dcsr &out, clb_spgemm_stats &st) {
     clb_controls_s *ctl = p.ctl;
+    one_pass_post(p);
     if (p.done) CLB_HIP(hipEventSynchronize(p.done), EVENT_WAITING_ERROR);
     p.tail = false;
     if (p.err.ptr && p.st.nzr) {

@@ -177,7 +177,9 @@ static void plan_run(clb_controls_s *ctl, clb_spgemm_plan_s &p, u32 k, bool timi
 
 // Every block in order, each handed to fn as soon as it is complete.  In
 // the one-pass modes block k + 1's passes start while block k's tail
-// (compaction and dense-row emission) still runs on the tail streams.
+// (compaction and dense-row emission) still runs on the tail streams, and
+// block k + 1's row kernels are launched before the host reads block k's
+// results back, so that they follow block k's on the device without a gap.
 static void plan_run_all(clb_controls_s *ctl, clb_spgemm_plan_s &p, bool timing, clb_spgemm_block_fn fn,
                          void *user) {
     if (ctl != p.ctl) raise(INVALID_ARGUMENT, "a plan runs on the controls it was created with");
@@ -199,7 +201,7 @@ static void plan_run_all(clb_controls_s *ctl, clb_spgemm_plan_s &p, bool timing,
         }
         return;
     }
-    OnePassPtr pend;
+    OnePassPtr pend, cur, nxt;  // pend: posted, its tail running; cur, nxt: launched
     u32 pend_k = 0;
     auto finish_pending = [&]() {
         clb_dcsr r{};
@@ -208,30 +210,52 @@ static void plan_run_all(clb_controls_s *ctl, clb_spgemm_plan_s &p, bool timing,
         pend.reset();
         deliver(pend_k, r, st);
     };
-    for (u32 k = 0; go && k + 1 < p.cuts.size(); ++k) {
-        if (pend) {
+    auto avail = [&]() -> u64 {
+        size_t free_b = 0, total_b = 0;
+        if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) (void)hipGetLastError();
+        return free_b + pool_for(ctl->device).cached_bytes();
+    };
+    auto launch = [&](u32 k) {
+        const PlanRows pr = rows_of(p, k, nullptr);
+        return one_pass_launch(ctl, p.a, p.b, p.cuts[k], p.cuts[k + 1], timing, p.mode == CLB_SPGEMM_BOUND, &pr);
+    };
+    u32 k = 0;
+    while (go && k + 1 < p.cuts.size()) {
+        if (!cur) {
             // overlap only when there is room for a second block of about
             // the pending one's size beside it (with room to spare for its
             // dense-row dump); otherwise let the pending block finish first
             // and hand its memory back to the pool
-            size_t free_b = 0, total_b = 0;
-            if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) (void)hipGetLastError();
-            const u64 avail = free_b + pool_for(ctl->device).cached_bytes();
-            if (avail < 3 * one_pass_bytes(*pend)) finish_pending();
+            if (pend && avail() < 3 * one_pass_bytes(*pend)) finish_pending();
+            cur = launch(k);
         }
-        OnePassPtr cur;
-        for (;;) {
-            bool replan = false;
-            PlanRows pr = rows_of(p, k, &replan);
-            cur = one_pass_start(ctl, p.a, p.b, p.cuts[k], p.cuts[k + 1], timing, p.mode == CLB_SPGEMM_BOUND, &pr);
-            if (!replan) break;
+        // launch block k + 1 before posting block k, so that its row kernels
+        // queue behind block k's on the side streams and the device does not
+        // wait for the host between the blocks: when there is room for it
+        // beside the pending block and block k with its output columns
+        if (go && k + 2 < p.cuts.size()) {
+            const PlanRows pr = rows_of(p, k, nullptr);
+            const u64 mine = one_pass_bytes(*cur) + 4 * std::min<u64>(pr.products, pr.bound);
+            const u64 need = std::max<u64>(pend ? one_pass_bytes(*pend) : 0, mine);
+            if (avail() >= 3 * need) nxt = launch(k + 1);
+        }
+        one_pass_post(*cur);
+        if (one_pass_replan(*cur)) {
+            // cut block k finer and start again from it (block k + 1's
+            // launch is dropped: its index moves with the new cuts)
+            nxt.reset();
             cur.reset();
             recut(ctl, p, k);
+            continue;
         }
         if (pend) finish_pending();
         pend = std::move(cur);
         pend_k = k;
+        cur = std::move(nxt);
+        ++k;
     }
+    nxt.reset();
+    cur.reset();  // launched after a stop: dropped
     if (pend) finish_pending();
 }
 

@@ -581,14 +581,20 @@ static void launch_bitmap(clb_controls_s *ctl, hipStream_t st, const Setup &s, c
     else launch_bitmap_c<WBITS, WG, MODE, false>(ctl, st, s, perm, n, o);
 }
 
+void join_deferred(clb_controls_s *ctl, const hipEvent_t *defer) {
+    for (int k = 0; k < clb_controls_s::NSIDE; ++k)
+        CLB_HIP(hipStreamWaitEvent(ctl->stream, defer[k], 0), EVENT_WAITING_ERROR);
+}
+
 void launch_bins(clb_controls_s *ctl, const Setup &s, const u32 *perm, const u32 off[NBINS + 1], int mode,
-                 const Out &o, LaunchTimer *lt, int family) {
-    fork_side(ctl);
+                 const Out &o, LaunchTimer *lt, int family, hipEvent_t *defer, bool one_stream) {
+    if (!one_stream) fork_side(ctl);
     // table kernels alternate over side streams 1.. (the first one takes
     // stream 0 when no bitmap kernel was launched there)
     int k = 0;
     auto next = [&]() {
         constexpr int NS = clb_controls_s::NSIDE;
+        if (one_stream) return ctl->stream;
         return k == 0 ? side_stream(ctl, k++) : side_stream(ctl, 1 + (k++ - 1) % (NS - 1));
     };
     LaunchTimer dummy;
@@ -610,7 +616,7 @@ void launch_bins(clb_controls_s *ctl, const Setup &s, const u32 *perm, const u32
         const u32 n = off[NBINS] - off[BIN_BITMAP0];
         if (n) {
             const u32 *pp = perm + off[BIN_BITMAP0];
-            hipStream_t st = side_stream(ctl, 0);
+            hipStream_t st = one_stream ? ctl->stream : side_stream(ctl, 0);
             k = 1;
             t.begin(st);
 #define BITMAP3(WB, WGS)                                                   \
@@ -699,7 +705,13 @@ void launch_bins(clb_controls_s *ctl, const Setup &s, const u32 *perm, const u32
         t.end(fam(1), st);
     }
 #undef DISPATCH3
-    join_side(ctl);
+    if (one_stream) return;
+    if (!defer) {
+        join_side(ctl);
+        return;
+    }
+    for (int j = 0; j < clb_controls_s::NSIDE; ++j)
+        CLB_HIP(hipEventRecord(defer[j], ctl->side[j]), EVENT_WAITING_ERROR);
 }
 
 struct Timer {

@@ -106,8 +106,14 @@ void sum_by_bin(clb_controls_s *ctl, const u32 *perm, const BinHost &h, u32 nrow
                 u64 out[NBINS]);
 // Launches the row kernels of every nonempty bin in `mode` (SYM, NUM or ONE)
 // on the side streams.  family < 0 times launches as sym_* / num_* by mode.
+// The controls' stream waits for them at the end, unless `defer` is given:
+// then the end of each side stream's work is recorded in defer[k] (NSIDE
+// events, made by the caller) for join_deferred.  one_stream: every kernel
+// on the controls' stream instead (no side streams).
 void launch_bins(clb_controls_s *ctl, const Setup &s, const u32 *perm, const u32 off[NBINS + 1], int mode,
-                 const Out &o, LaunchTimer *lt, int family);
+                 const Out &o, LaunchTimer *lt, int family, hipEvent_t *defer = nullptr, bool one_stream = false);
+// The controls' stream waits for the work recorded by a deferred launch_bins.
+void join_deferred(clb_controls_s *ctl, const hipEvent_t *defer);
 void scan_sizes(clb_controls_s *ctl, Setup &s, const u32 *nnz_row, DevArray<u64> &off, DevArray<u32> &pos);
 void assemble(clb_controls_s *ctl, const Setup &s, const clb_dcsr &a, const u32 *nnz_row, const u64 *off,
               const u32 *pos, u64 nnz, u32 nzr, clb_buffer_s *cols, clb_dcsr &out);
@@ -129,13 +135,21 @@ void spgemm_learned(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &b, u
 // tail (compaction and dense-row emission), which it leaves running on the
 // controls' tail streams; finish waits for it and hands over the result.  A
 // block that exceeded a plan's bound (one_pass_replan) has no result.
+// start is itself launch (binning, and the row kernels left running on the
+// side streams) followed by post (the rest, from the overflow count on), so
+// that a caller can launch the next block before posting this one: its row
+// kernels then queue behind this block's on the side streams and start as
+// soon as they end, while the host posts this block.
 struct OnePass;
 struct OnePassDeleter {
-    void operator()(OnePass *p) const;  // waits for the block's tail first
+    void operator()(OnePass *p) const;  // waits for the block's kernels first
 };
 using OnePassPtr = std::unique_ptr<OnePass, OnePassDeleter>;
 OnePassPtr one_pass_start(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &b, u32 row_begin, u32 row_end,
                           bool timing, bool bound_only, const PlanRows *pr);
+OnePassPtr one_pass_launch(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcsr &b, u32 row_begin, u32 row_end,
+                           bool timing, bool bound_only, const PlanRows *pr);
+void one_pass_post(OnePass &p);
 bool one_pass_replan(const OnePass &p);
 u64 one_pass_bytes(const OnePass &p);  // device memory the block holds
 void one_pass_finish(OnePass &p, clb_dcsr &out, clb_spgemm_stats &st);
