@@ -332,20 +332,9 @@ clb_status clb_closure_step(clb_controls ctl, clb_dcsr *mats, uint32_t nmats, co
                         splits.push_back(hs);
                     };
                     const bool one_pass = so.mode == CLB_SPGEMM_LEARNED || so.mode == CLB_SPGEMM_BOUND;
-                    // one-pass blocks overlap: block k + 1's passes start while
-                    // block k's compaction and dense-row emission still run
-                    // (when device memory allows two blocks at once)
-                    spg::OnePassPtr pend;
-                    auto finish_pending = [&]() {
-                        clb_dcsr T{};
-                        clb_spgemm_stats ss;
-                        spg::one_pass_finish(*pend, T, ss);
-                        pend.reset();
-                        absorb(T, ss);
-                    };
-                    for (u32 k = 0; k < nb; ++k) {
-                        if (hc[k + 1] <= hc[k]) continue;
-                        if (!one_pass) {
+                    if (!one_pass) {
+                        for (u32 k = 0; k < nb; ++k) {
+                            if (hc[k + 1] <= hc[k]) continue;
                             clb_spgemm_opts bo = so;
                             bo.row_begin = hc[k];
                             bo.row_end = hc[k + 1];
@@ -353,20 +342,53 @@ clb_status clb_closure_step(clb_controls ctl, clb_dcsr *mats, uint32_t nmats, co
                             clb_dcsr T{};
                             spg::spgemm_dispatch(ctl, Y, Z, bo, T, ss);
                             absorb(T, ss);
-                            continue;
                         }
-                        if (pend) {
+                    } else {
+                        // one-pass blocks overlap: block k + 1's passes start while
+                        // block k's compaction and dense-row emission still run,
+                        // and its row kernels are launched before block k is
+                        // posted, so that they follow block k's on the device
+                        // (each when device memory allows)
+                        spg::OnePassPtr pend, cur, nxt;
+                        auto finish_pending = [&]() {
+                            clb_dcsr T{};
+                            clb_spgemm_stats ss;
+                            spg::one_pass_finish(*pend, T, ss);
+                            pend.reset();
+                            absorb(T, ss);
+                        };
+                        auto avail = [&]() -> u64 {
                             size_t free_b = 0, total_b = 0;
                             if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) (void)hipGetLastError();
-                            if (free_b + pool_for(ctl->device).cached_bytes() < 3 * spg::one_pass_bytes(*pend))
-                                finish_pending();
+                            return free_b + pool_for(ctl->device).cached_bytes();
+                        };
+                        auto launch = [&](u32 k) {
+                            const bool bound_only = so.mode == CLB_SPGEMM_BOUND;
+                            return spg::one_pass_launch(ctl, Y, Z, hc[k], hc[k + 1], false, bound_only, nullptr);
+                        };
+                        auto nonempty = [&](u32 k) {  // the first block from k with rows
+                            while (k < nb && hc[k + 1] <= hc[k]) ++k;
+                            return k;
+                        };
+                        for (u32 k = nonempty(0); k < nb;) {
+                            if (!cur) {
+                                if (pend && avail() < 3 * spg::one_pass_bytes(*pend)) finish_pending();
+                                cur = launch(k);
+                            }
+                            const u32 k2 = nonempty(k + 1);
+                            if (k2 < nb) {
+                                const u64 mine = spg::one_pass_bytes(*cur) + 4 * hb[k];
+                                const u64 need = std::max<u64>(pend ? spg::one_pass_bytes(*pend) : 0, mine);
+                                if (avail() >= 3 * need) nxt = launch(k2);
+                            }
+                            spg::one_pass_post(*cur);
+                            if (pend) finish_pending();
+                            pend = std::move(cur);
+                            cur = std::move(nxt);
+                            k = k2;
                         }
-                        spg::OnePassPtr cur = spg::one_pass_start(ctl, Y, Z, hc[k], hc[k + 1], false,
-                                                                  so.mode == CLB_SPGEMM_BOUND, nullptr);
                         if (pend) finish_pending();
-                        pend = std::move(cur);
                     }
-                    if (pend) finish_pending();
                 }
             } catch (...) {
                 release_all();

@@ -797,7 +797,8 @@ __device__ __forceinline__ void bitmap_row(const Mats &g, const u32 *__restrict_
         } else {
             // only the slice's part of each B row is read (with the split table)
             if constexpr (COMB) {
-                if (use_splits) for_each_product<WG, UBW, LONG_ROW, true>(g, js, je, big, setu, setdw, s, sstep, &ah, sn);
+                if (use_splits)
+                    for_each_product<WG, UBW, LONG_ROW, true>(g, js, je, big, setu, setdw, s, sstep, &ah, sn);
                 else for_each_product<WG, UBW, LONG_ROW, true>(g, js, je, big, setd, NOSLICE, sstep);
             } else {
                 if (use_splits) for_each_product<WG, UBW, LONG_ROW, true>(g, js, je, big, setu, s, sstep, &ah, sn);
