@@ -182,7 +182,7 @@ __device__ __forceinline__ void for_each_product(const Mats &g, u32 js, u32 je, 
     constexpr int NW = WG / dev::WAVE;
     constexpr u32 WIN = UB * 64;
     // AHEAD (a metadata pipeline, for kernels whose occupancy LDS limits:
-    // it costs ~7 VGPRs): `part` holds the B part of this lane's nonzero in
+    // it costs 7 to 18 VGPRs): `part` holds the B part of this lane's nonzero in
     // the current chunk and `knext` the A column of its nonzero in the next
     // chunk.  Before its gather, a chunk issues the loads of the next chunk's
     // part and of the column two chunks ahead (the first chunk, given a
@@ -767,8 +767,11 @@ __device__ __forceinline__ void bitmap_row(const Mats &g, const u32 *__restrict_
     u32 slot = 0;          // its slot in the dump
     u32 *out = MODE == SYM ? nullptr : o.cols + o.off[r];
     const u64 lim = row_limit<MODE>(o, r);
+    // the metadata lookahead costs ~16 VGPRs: only where LDS already holds
+    // occupancy to two 512-thread (or one 1024-thread) workgroups per CU
+    constexpr bool AH = WBITS >= 14;
     RowAhead ah{EMPTY, make_uint2(0u, 0u)};
-    if (use_splits && nslices > 1) {
+    if (AH && use_splits && nslices > 1) {
         if (threadIdx.x < je - js) ah.k0 = g.a_cols[js + threadIdx.x];
         ah.part = b_part(g, ah.k0, 0, sstep);
     }
@@ -792,17 +795,17 @@ __device__ __forceinline__ void bitmap_row(const Mats &g, const u32 *__restrict_
         auto setdw = [&](u32 col) { dev::wave_or_sorted_bits(bm, (u32)((u64)col - base)); };
         constexpr int UBW = MODE == SYM ? 16 : 8;
         if (nslices == 1) {
-            if constexpr (COMB) for_each_product<WG, UBW, LONG_ROW, true>(g, js, je, big, set1, setw);
-            else for_each_product<WG, UBW, LONG_ROW, true>(g, js, je, big, set1);
+            if constexpr (COMB) for_each_product<WG, UBW, LONG_ROW, AH>(g, js, je, big, set1, setw);
+            else for_each_product<WG, UBW, LONG_ROW, AH>(g, js, je, big, set1);
         } else {
             // only the slice's part of each B row is read (with the split table)
             if constexpr (COMB) {
                 if (use_splits)
-                    for_each_product<WG, UBW, LONG_ROW, true>(g, js, je, big, setu, setdw, s, sstep, &ah, sn);
-                else for_each_product<WG, UBW, LONG_ROW, true>(g, js, je, big, setd, NOSLICE, sstep);
+                    for_each_product<WG, UBW, LONG_ROW, AH>(g, js, je, big, setu, setdw, s, sstep, &ah, sn);
+                else for_each_product<WG, UBW, LONG_ROW, AH>(g, js, je, big, setd, NOSLICE, sstep);
             } else {
-                if (use_splits) for_each_product<WG, UBW, LONG_ROW, true>(g, js, je, big, setu, s, sstep, &ah, sn);
-                else for_each_product<WG, UBW, LONG_ROW, true>(g, js, je, big, setd, NOSLICE, sstep);
+                if (use_splits) for_each_product<WG, UBW, LONG_ROW, AH>(g, js, je, big, setu, s, sstep, &ah, sn);
+                else for_each_product<WG, UBW, LONG_ROW, AH>(g, js, je, big, setd, NOSLICE, sstep);
             }
         }
         __syncthreads();
