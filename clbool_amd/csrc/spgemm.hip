@@ -354,9 +354,11 @@ BinCfg make_bincfg(const clb_controls_s *ctl, u32 ncols) {
     // per CU); wider matrices are processed in column slices.  The kernels
     // that also write columns use 2^14 words (64 KB, two 512-thread
     // workgroups per CU, so one row's emission overlaps the other's gather)
-    // when that costs at most one extra slice, i.e. up to 2^20 columns; on
-    // wider matrices the extra slices (each re-reads the A row and its B-row
-    // pieces) cost more than the overlap gains.
+    // when that costs at most one extra slice, i.e. up to 2^20 columns.  On
+    // wider matrices each extra slice re-reads the A row and its B-row
+    // pieces; whether the overlap still pays depends on the rows (with
+    // dense-row dumps and the ESC bin it does at R-MAT scale 22), so the
+    // bitmap_window_bits knob can force the window.
     const u64 words = ((u64)ncols + 31) / 32;
     u32 wb = 6;
     while (wb < 15 && (1ull << wb) < words) ++wb;

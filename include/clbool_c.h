@@ -322,8 +322,10 @@ clb_status clb_spgemm_plan_run(clb_controls ctl, clb_spgemm_plan plan, uint32_t 
  * complete: fn(user, k, &block, &stats) owns the block (it releases it with
  * clb_dcsr_release) and returns 0 to go on, nonzero to stop.  In the one-pass
  * modes block k + 1's passes start while block k's last phase (compaction and
- * dense-row emission) still runs, so the blocks overlap; fn is called on the
- * calling thread.  Results are the same as clb_spgemm_plan_run's. */
+ * dense-row emission) still runs, and block k + 1's row kernels are launched
+ * before block k's sizes are read back, so the blocks overlap (each when
+ * device memory allows); fn is called on the calling thread.  Results are the
+ * same as clb_spgemm_plan_run's. */
 typedef int32_t (*clb_spgemm_block_fn)(void *user, uint32_t k, clb_dcsr *block, const clb_spgemm_stats *stats);
 clb_status clb_spgemm_plan_run_all(clb_controls ctl, clb_spgemm_plan plan, uint32_t timing, clb_spgemm_block_fn fn,
                                    void *user);
