@@ -522,8 +522,7 @@ struct OnePass {
     clb_dcsr a{}, b{};
     PlanRows prc{};
     bool planned = false, posted = false, empty = false;
-    u32 *cap = nullptr;
-    const float *pred = nullptr;
+    const float *pred = nullptr;  // predicted sizes (compaction's error statistic)
     std::chrono::steady_clock::time_point t0;
     ~OnePass() {
         for (hipEvent_t &e : bins)
@@ -591,7 +590,6 @@ OnePassPtr one_pass_launch(clb_controls_s *ctl, const clb_dcsr &a, const clb_dcs
     st.rows = nrows;
     st.n_bins = NBINS;
     st.capacity = sc[4];
-    p.cap = cap;
     p.pred = pred;
     if (st.products == 0) {
         p.empty = true;
