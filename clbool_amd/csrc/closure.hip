@@ -379,7 +379,7 @@ clb_status clb_closure_step(clb_controls ctl, clb_dcsr *mats, uint32_t nmats, co
                             if (k2 < nb) {
                                 const u64 mine = spg::one_pass_bytes(*cur) + 4 * hb[k];
                                 const u64 need = std::max<u64>(pend ? spg::one_pass_bytes(*pend) : 0, mine);
-                                if (avail() >= 3 * need) nxt = launch(k2);
+                                if (avail() >= 2 * need) nxt = launch(k2);
                             }
                             spg::one_pass_post(*cur);
                             if (pend) finish_pending();

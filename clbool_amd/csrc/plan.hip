@@ -231,13 +231,15 @@ static void plan_run_all(clb_controls_s *ctl, clb_spgemm_plan_s &p, bool timing,
         }
         // launch block k + 1 before posting block k, so that its row kernels
         // queue behind block k's on the side streams and the device does not
-        // wait for the host between the blocks: when there is room for it
-        // beside the pending block and block k with its output columns
+        // wait for the host between the blocks: when free memory holds twice
+        // the larger of the pending block and block k with its output columns
+        // (one such block for k + 1, and as much again for block k's output
+        // and k + 1's later allocations)
         if (go && k + 2 < p.cuts.size()) {
             const PlanRows pr = rows_of(p, k, nullptr);
             const u64 mine = one_pass_bytes(*cur) + 4 * std::min<u64>(pr.products, pr.bound);
             const u64 need = std::max<u64>(pend ? one_pass_bytes(*pend) : 0, mine);
-            if (avail() >= 3 * need) nxt = launch(k + 1);
+            if (avail() >= 2 * need) nxt = launch(k + 1);
         }
         one_pass_post(*cur);
         if (one_pass_replan(*cur)) {
