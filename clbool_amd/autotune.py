@@ -43,6 +43,10 @@ for _b in range(7, 14):
 RESERVED = {"bitmap_pred_caps": 0, "gather_combine": 1, "esc_max_products": 2, "compact_tiles": 3,
             "onepass_bitmap_min": 5, "bitmap_window_bits": 6, "emit_sparse_bits": 7}
 RESERVED_BITS = {f"hash_wide_{b}": (4, b - 7) for b in range(7, 14)}
+# on matrices wider than one bitmap window: rows within the largest hash table
+# take a table rather than a bitmap of several column slices (1)
+KNOBS["tables_over_slices"] = [0, 1]
+RESERVED_BITS["tables_over_slices"] = (4, 16)
 LEARNED_ONLY = ("learned_margin_pct", "bitmap_pred_caps")   # knobs of the predictor's sizing
 ONE_PASS_ONLY = ("compact_tiles", "onepass_bitmap_min")      # knobs of the learned and bound one pass
 

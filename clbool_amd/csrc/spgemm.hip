@@ -367,6 +367,7 @@ BinCfg make_bincfg(const clb_controls_s *ctl, u32 ncols) {
     const u32 forced = ctl->tuning.reserved[CLB_TUNE_BITMAP_WINDOW_BITS];
     if (forced >= 13 && forced <= 15) c.bm_bits = forced < wb ? forced : wb;
     c.multi_slice = (u64)ncols > (32ull << c.bm_bits) ? 1u : 0u;
+    c.tables_over_slices = (ctl->tuning.reserved[CLB_TUNE_HASH_WIDE] >> CLB_TUNE_TABLES_OVER_SLICES_BIT) & 1u;
     const u32 em = ctl->tuning.reserved[CLB_TUNE_ESC_MAX_PRODUCTS];
     c.esc_max = em < (1u << ESC_MAX_BITS) ? em : (1u << ESC_MAX_BITS);
     c.esc_only = 0;

@@ -120,6 +120,7 @@ struct BinCfg {
     // have at most esc_max products (0: never)
     u32 esc_max;
     u32 multi_slice;  // ncols spans more than one bitmap window
+    u32 tables_over_slices;  // multi_slice: the slot-ratio rule does not send rows to a bitmap
     // the ESC SpGEMM (dcsr::matrix_multiplication): tiny, LDS ESC up to
     // 2^ESC_MAX_BITS products, longer rows to the global sort (BIN_BITMAP0)
     u32 esc_only;
@@ -139,8 +140,12 @@ __device__ __forceinline__ u32 classify(u32 P, u32 key, const BinCfg &c) {
     if (c.esc_only) return P <= (1u << ESC_MAX_BITS) ? esc_bin(P) : BIN_BITMAP0;
     u32 bits = ceil_log2(2 * key);
     if (bits < HASH_MIN_BITS) bits = HASH_MIN_BITS;
+    // the slot-ratio rule (a bitmap no larger than ratio x the table costs
+    // no more) assumes one window; with tables_over_slices a row that would
+    // need several column slices takes the table instead
+    const bool ratio_rule = !(c.multi_slice && c.tables_over_slices);
     const bool bitmap = P >= c.bitmap_min || bits > c.hash_max_bits ||
-                        (1ull << c.bm_bits) <= (u64)c.bitmap_ratio * (1ull << bits);
+                        (ratio_rule && (1ull << c.bm_bits) <= (u64)c.bitmap_ratio * (1ull << bits));
     if (!bitmap) return BIN_HASH0 + (bits - HASH_MIN_BITS);
     if (c.multi_slice && P <= c.esc_max) return esc_bin(P);
     const u32 lp = 31 - __clz(P);

@@ -64,7 +64,8 @@ typedef struct clb_tuning {
     uint32_t bitmap_min_products;  /* rows with at least this many products use
                                       the LDS bitmap accumulator                  */
     uint32_t bitmap_slot_ratio;    /* use a bitmap when its words <= ratio * the
-                                      hash slots the row would need               */
+                                      hash slots the row would need (when the
+                                      columns fit one bitmap window)              */
     uint32_t tiny_max_products;    /* rows up to this many products use the
                                       wave-level sort kernel                      */
     uint32_t learned_margin_pct;   /* learned sizing: capacity = pred * (100 + m)%  */
@@ -105,7 +106,11 @@ typedef struct clb_tuning {
                                       twice the default workgroup (64 threads
                                       for 2^7..2^9 slots, 128 for 2^10..2^11,
                                       256 for 2^12, 512 for 2^13; 2^14: 1024
-                                      either way)
+                                      either way); bit
+                                      CLB_TUNE_TABLES_OVER_SLICES_BIT set: when
+                                      the columns span several bitmap windows,
+                                      bitmap_slot_ratio sends no row to a bitmap
+                                      (rows within hash_max_slots take a table)
                                       reserved[CLB_TUNE_ONEPASS_BITMAP_MIN]: in
                                       the one pass, rows with at least this many
                                       products use a bitmap whatever their
@@ -130,6 +135,7 @@ typedef struct clb_tuning {
 #define CLB_TUNE_ESC_MAX_PRODUCTS 2
 #define CLB_TUNE_COMPACT_TILES    3
 #define CLB_TUNE_HASH_WIDE        4
+#define CLB_TUNE_TABLES_OVER_SLICES_BIT 16  /* a bit of reserved[CLB_TUNE_HASH_WIDE] */
 #define CLB_TUNE_ONEPASS_BITMAP_MIN 5
 #define CLB_TUNE_BITMAP_WINDOW_BITS 6
 #define CLB_TUNE_EMIT_SPARSE_BITS   7

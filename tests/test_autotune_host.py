@@ -38,9 +38,12 @@ def test_knobs_round_trip_including_reserved_slots():
         at._set(t, k, vals[-1])
     d = {k: at._get(t, k) for k in at.KNOBS}
     assert d == {k: v[-1] for k, v in at.KNOBS.items()}
-    # the workgroup-shape bits share one word without disturbing each other
-    assert t.reserved[4] == 0x7F
+    # the workgroup-shape bits and the tables-over-slices bit share one word
+    # without disturbing each other
+    assert t.reserved[4] == 0x1007F
     at._set(t, "hash_wide_9", 0)
-    assert t.reserved[4] == 0x7B and at._get(t, "hash_wide_10") == 1
+    assert t.reserved[4] == 0x1007B and at._get(t, "hash_wide_10") == 1
+    at._set(t, "tables_over_slices", 0)
+    assert t.reserved[4] == 0x7B and at._get(t, "hash_wide_13") == 1
     assert t.reserved[at.RESERVED["bitmap_pred_caps"]] == at.KNOBS["bitmap_pred_caps"][-1]
     assert t.reserved[at.RESERVED["gather_combine"]] == at.KNOBS["gather_combine"][-1]

@@ -112,6 +112,9 @@ def test_hypersparse_2p24(controls):
          emit_sparse_bits=1),
     dict(hash_max_slots=16384, bitmap_min_products=1, bitmap_slot_ratio=1, tiny_max_products=0,
          emit_sparse_bits=1 << 20, dump_bits_per_chunk=65536),
+    # the ratio rule's bitmap rows of the wide case go to the largest tables
+    dict(hash_max_slots=16384, bitmap_min_products=1 << 30, bitmap_slot_ratio=4, tiny_max_products=0,
+         tables_over_slices=1),
 ])
 def test_every_kernel_path(controls, knobs):
     """Force the rows through each bin family (hash only, tiny, bitmap only)."""
