@@ -276,7 +276,7 @@ def _dcsr_rows(rpt, rows, cols, keep):
     return np.r_[0, np.cumsum(lens)].astype(np.uint32), rows[keep].astype(np.uint32), cols[idx].astype(np.uint32)
 
 
-def bench_c5(cb, gen, ctl, log, scale=22, ef=32, step_scale=16, seed=1):
+def bench_c5(cb, gen, ctl, log, scale=22, ef=32, step_scale=16, seed=1, tunings=None):
     """The RPQ input of a 64x64 automaton M (200 uniform nonzeros) and an
     R-MAT graph G (scale 22, edge factor 32).  K = M (x) G has about 2.6e10
     nonzeros, more than one uint32 DCSR holds, so it is built in blocks of
@@ -285,7 +285,8 @@ def bench_c5(cb, gen, ctl, log, scale=22, ef=32, step_scale=16, seed=1):
     about 3e14 products, beyond one GPU; the step itself is measured on the
     same automaton with G at scale `step_scale` (K of width 64 * 2^16 = 2^22),
     whole, in row blocks of output bound below 2^32, plain and learned; both
-    results must agree."""
+    results must agree.  tunings: knobs per mode ("plain", "learned") applied
+    before that mode's step (None: the controls' current knobs for both)."""
     mr, mc = gen.uniform_pairs(200, 64, 64, seed)
     mrpt, mrows, mcols = gen.to_dcsr(mr, mc)
     out = {"automaton": {"n": 64, "nnz": int(mr.size), "seed": seed}}
@@ -339,6 +340,9 @@ def bench_c5(cb, gen, ctl, log, scale=22, ef=32, step_scale=16, seed=1):
     train = train_predictor(cb, gen, ctl, log)
     prints = {}
     for name, mode in (("plain", cb.SPGEMM_PLAIN), ("learned", cb.SPGEMM_LEARNED)):
+        if tunings and name in tunings:
+            from clbool_amd import autotune as at
+            at.apply(ctl, tunings[name])
         best = None
         for rep in range(3):
             ctl.sync()
@@ -733,8 +737,10 @@ def main():
         out["hypersparse"] = bench_hypersparse(cb, gen, ctl, args.hypersparse, log)
     if not args.no_c5 and "learned" in modes:
         ctl.trim()
-        at.apply(ctl, tuned.get("learned", base_tuning))
-        out["c5"] = bench_c5(cb, gen, ctl, log)
+        # each mode's step with that mode's knobs of the main workload (K is
+        # 2^22 columns wide, as R-MAT 22)
+        out["c5"] = bench_c5(cb, gen, ctl, log,
+                             tunings={m: tuned.get(m, base_tuning) for m in ("plain", "learned")})
     if not args.no_cpu_baseline:
         log("[bench] timing the CPU baseline")
         out["cpu_baseline"] = cpu_baseline(oracle, A)
